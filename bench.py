@@ -1,0 +1,151 @@
+"""Benchmark: affected-vertex updates/s of the plain one-graph GPU loop.
+
+Workload (one "step"): the full 10-batch DynLP update stream of a
+500,000-vertex, average-degree-7 synthetic graph (generator seed 42, batch
+seed 42+12345, tau = mean edge weight, delta 1e-4, maxIterations 100000),
+replayed from the pre-batch state on device-resident inputs.  The graph and
+batch lists are uploaded before the timed region; the state reset (device
+fills) is part of each step.
+
+value = U / step time, U = sum over batches and iterations of the frontier
+size (the reference's numAffected), identical for every implementation that
+matches the reference's results.
+
+Prints ONE JSON line.  --gpus N with N > 1 exits: this build runs on one GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+
+def propagation_bytes(prof: dict, updates: int) -> int:
+    """Algorithmic bytes of the propagation launches (4-byte ids, weights,
+    labels): per frontier vertex 20 (queue id 4, row start 4, row length 4,
+    F_old[u] 4, F_new[u] 4) + 12 per row entry (column 4, weight 4,
+    F_old[v] 4); per changed vertex 4 per stamp update of its row entries and
+    itself; 4 per next-frontier append."""
+    return 20 * updates + 12 * prof["edges"] + 4 * prof["changedEdges"] + 4 * prof["appended"]
+
+
+def cpu_baseline(inputs, budget_s: float = 40.0):
+    """The CPU oracle (single thread) on the same stream, batch by batch,
+    stopping once the time budget is spent; returns updates/s over the
+    batches run."""
+    from oracle.oracle import Oracle
+    orc = Oracle(inputs)
+    done, upd, t = 0, 0, 0.0
+    for b in range(inputs.numBatches):
+        t0 = time.perf_counter()
+        orc.run_batch(b)
+        t += time.perf_counter() - t0
+        upd += orc.stats()["updates"]
+        done += 1
+        if t > budget_s:
+            break
+    return upd / t, done, t
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--nodes", type=int, default=500000)
+    ap.add_argument("--degree", type=int, default=7)
+    ap.add_argument("--batches", type=int, default=10)
+    ap.add_argument("--lanes", type=int, default=0, help="propagation lanes per vertex")
+    ap.add_argument("--blocks", type=int, default=0, help="propagation blocks per launch")
+    ap.add_argument("--chunk", type=int, default=0, help="iterations between host checks")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print(json.dumps({"error": "this build measures one GPU; run with --gpus 1"}))
+        return 2
+
+    import labelpropcuda_amd as L
+    if L.device_count() < 1:
+        print(json.dumps({"error": "no GPU visible"}))
+        return 1
+
+    inputs = L.generate(args.nodes, args.degree, args.batches, 42)
+    cfg = L.make_config(inputs)
+    ctx = L.Context(inputs, cfg)
+    if args.lanes or args.blocks or args.chunk:
+        ctx.set_tuning(args.lanes, args.blocks, args.chunk)
+
+    for _ in range(args.warmup):
+        ctx.run_all()
+    ctx.synchronize()
+
+    ctx.set_profiling(True)
+    lat, updates = [], None
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ctx.run_all()
+        ctx.synchronize()
+        st = [ctx.stats(b) for b in range(inputs.numBatches)]
+        lat.extend(s["timeMs"] for s in st)
+        u = sum(s["updates"] for s in st)
+        assert updates is None or u == updates, "update count changed between steps"
+        updates = u
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    prof = ctx.prop_profile()
+    ctx.close()
+
+    ms_per_step = 1000.0 * wall / args.steps
+    value = updates / (wall / args.steps)
+    launches = max(prof["launches"], 1)
+    bytes_per_launch = propagation_bytes(prof, updates * args.steps) / launches
+    avg_launch_s = prof["kernelMs"] / 1000.0 / launches
+    achieved = bytes_per_launch / avg_launch_s / 1e9
+
+    out = {
+        "metric": "affected-vertex updates/s (plain 1-GPU loop)",
+        "value": value,
+        "unit": "updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (reference generator, seed 42)",
+        "config": {"workload": "DynLP plain loop, 500000 vertices, avg degree 7, 10 batches "
+                               "(90/1/9), tau=mean weight, delta 1e-4",
+                   "vertices": args.nodes, "avg_degree": args.degree, "batches": args.batches,
+                   "updates_per_step": updates},
+        "batch_latency_ms": {"p50": float(np.percentile(lat, 50)),
+                             "p99": float(np.percentile(lat, 99))},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_propagate", "launches_per_step": prof["launches"] / args.steps,
+                     "avg_launch_us": avg_launch_s * 1e6,
+                     "algorithmic_bytes_per_launch": bytes_per_launch},
+    }
+    if not args.no_cpu:
+        v, done, t = cpu_baseline(inputs)
+        out["cpu_baseline"] = {"value": v, "unit": "updates/s", "cores": 1, "kind": "port",
+                               "sample": f"first {done} of {inputs.numBatches} batches of the "
+                                         f"same stream, {t:.1f} s, CPU oracle (single thread)"}
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,307 @@
+/*
+ * dynlp_c.h -- C-ABI of the MI355X DynLP batch-update library (libdynlp.so).
+ *
+ * Plain C: pointers, sizes and int status codes; no GPU-runtime or C++ types.
+ * `void* stream` is a hipStream_t (NULL = the default stream).  Every entry
+ * point returns DLP_OK (0) or a DLP_ERR_* code; dlp_last_error() gives the
+ * message of the last failure on the calling thread.
+ *
+ * Three groups of entry points:
+ *   1. Step API -- one entry per reference step launcher, same argument order
+ *      and meaning; raw device pointers; host scalars that the reference
+ *      returns through `int&` come back through `int*` (these calls
+ *      synchronise `stream`, as the reference's do).
+ *   2. runDynLP -- the reference's batch loop over host inputs.
+ *   3. Context API -- the same batch loop with the graph and the whole batch
+ *      stream resident in HBM, for replay, tracing and measurement.
+ *   4. Input generation -- the reference's synthetic graph and batch
+ *      generators (host code), so that callers get identical inputs.
+ *
+ * Precision contract: FP32 with IEEE division and no FMA contraction; the
+ * propagation sum follows the reference kernel's 256-slot block reduction
+ * tree position for position, so results are bit-identical to the CPU
+ * restatement that follows the same order.
+ */
+#ifndef DYNLP_C_H
+#define DYNLP_C_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------- */
+#define DLP_OK 0
+#define DLP_ERR_INVALID 1   /* bad argument / precondition violated */
+#define DLP_ERR_HIP 2       /* HIP runtime error                    */
+#define DLP_ERR_NOMEM 3     /* device or host allocation failed     */
+#define DLP_ERR_STATE 4     /* call out of order / bad handle       */
+#define DLP_ERR_NODEVICE 5  /* no GPU visible                        */
+#define DLP_ERR_INTERNAL 6  /* a device-side bounded wait expired    */
+
+/* ---- vertex status values (reference graphTypes.h:191-195) ----------- */
+#define DLP_VERTEX_ACTIVE 0
+#define DLP_VERTEX_DELETED 1
+#define DLP_VERTEX_GROUND_TRUTH 2
+
+/* Message of the last failing call on this thread ("" if none). */
+const char *dlp_last_error(void);
+
+/* Library version string. */
+const char *dlp_version(void);
+
+/* Number of visible GPUs (0 when none; never an error). */
+int dlp_device_count(int *count);
+
+/* ====================================================================== */
+/* 1. Step API                                                            */
+/* ====================================================================== */
+
+/* Replaces launchMarkDeletions (sparsifySubgraph.h:41-43, .cu:185-203).
+ * For each deleted u: isAffected[v] = 1 for every valid v in row(u), row(u)
+ * entries set to -1, vertexStatus[u] = DELETED; then every entry of a
+ * non-deleted row that points at a DELETED vertex is set to -1. */
+int dlp_mark_deletions(int *rowPtr, int *colInd, int *vertexStatus,
+                       int *isAffected, const int *deletedNodes,
+                       int numDeleted, int numNodes, void *stream);
+
+/* Replaces launchBuildSparsifiedSubgraph (sparsifySubgraph.h:78-84,
+ * .cu:205-287).  Edges with w > tau and both ends new go to G' in both
+ * directions, in input order: (u,v,w) then (v,u,w).  subColInd / subValues
+ * receive the destinations / weights like the reference; subEdgeSrc
+ * (nullable, not in the reference signature) receives the sources, which the
+ * reference frees before its CC step.  Endpoints u of every w > tau edge
+ * are marked affected iff 0 <= u < numInserted (the reference bounds marks
+ * by numSubNodes == numInserted, .cu:177-178, :259-263).  subRowPtr is not
+ * written (the reference never writes it either).  Output arrays need room
+ * for 2 * numNewEdges entries.  Synchronises `stream`. */
+int dlp_build_sparsified_subgraph(
+    int *rowPtr, int *colInd, float *values, int *isAffected,
+    const int *vertexStatus, const int *newEdgeSrc, const int *newEdgeDst,
+    const float *newEdgeWeight, int numNewEdges, const int *insertedNodes,
+    int numInserted, const int *isNewVertex, float tau, int *subRowPtr,
+    int *subColInd, float *subValues, int *subEdgeSrc, int *numSubNodes,
+    int *numSubEdges, void *stream);
+
+/* Replaces launchConnectedComponents (connectedComponents.h:50-54,
+ * .cu:162-290).  Components of the inserted vertices under the directed
+ * edge list (edgeSrc[i], edgeDst[i]) -- G', per the header contract
+ * connectedComponents.h:33-36.  componentId[v] = rank of the component's
+ * minimum vertex id among all component minima (ascending), -1 for vertices
+ * not inserted.  numEdges == 0 follows the reference (.cu:191-205): all
+ * componentIds -1 and numComponents = numInserted.  Edge endpoints must be
+ * inserted vertices.  Synchronises `stream`. */
+int dlp_connected_components(const int *edgeSrc, const int *edgeDst,
+                             int numEdges, const int *insertedNodes,
+                             int numInserted, int *componentId,
+                             int *numComponents, int numNodes, void *stream);
+
+/* Replaces launchLabelInitialization (labelInitialization.h:54-60,
+ * .cu:148-187).  W0[c] / W1[c] = weight from component c to ground-truth
+ * class 0 / 1 vertices; F_u = W1/(W0+W1) if W0+W1 > 1e-10f else 0.5;
+ * ground truth -> 0 / 1; componentId < 0 -> 0.5.  Deterministic order: each
+ * member's row is summed left to right, then members' sums are combined by a
+ * bottom-up pairwise tree in ascending vertex id.  insertedNodes must be
+ * distinct. */
+int dlp_label_initialization(const int *rowPtr, const int *colInd,
+                             const float *values, float *labels,
+                             const int *vertexStatus, const int *componentId,
+                             int numComponents, const int *insertedNodes,
+                             int numInserted, const int *gtClass,
+                             int numNodes, void *stream);
+
+/* Replaces launchIterativePropagation (iterativePropagation.h:73-80,
+ * .cu:241-346).  Frontier = {v : isAffected[v] == 1, status ACTIVE, not
+ * ground truth}; each iteration F_new[u] = clamp(sum w F_old / sum w) over
+ * non-deleted neighbours (F_old[u] if the weight sum is <= 1e-10f); a vertex
+ * whose label moved by more than delta re-enters the next frontier with its
+ * eligible neighbours.  Double-buffered exactly like the reference: labels
+ * and labelsBuffer alternate, non-frontier entries of the write buffer are
+ * not refreshed, and the last written buffer ends in `labels`, with ground
+ * truth enforced.  isAffected holds the reference's exit state.
+ * Synchronises `stream`. */
+int dlp_iterative_propagation(const int *rowPtr, const int *colInd,
+                              const float *values, float *labels,
+                              float *labelsBuffer, const int *vertexStatus,
+                              const int *gtClass, int *isAffected,
+                              int numNodes, float delta, int maxIterations,
+                              int *iterationsUsed, void *stream);
+
+/* ====================================================================== */
+/* 2. runDynLP over host inputs                                           */
+/* ====================================================================== */
+
+/* DynLPConfig minus its output-directory string (graphTypes.h:124-154). */
+typedef struct dlp_config {
+  int totalNodes;
+  int avgDegree;
+  int numBatches;
+  unsigned int seed;
+  float tau;  /* < 0: auto (mean full-graph edge weight, dynLP.cu:186-190) */
+  float delta;
+  int maxIterations;
+  float groundTruthFraction;
+  float unlabeledFraction;
+  float deletedFraction;
+  int validate;
+  int verbose;
+} dlp_config;
+
+/* Reference defaults (graphTypes.h:147-153). */
+void dlp_config_default(dlp_config *cfg);
+
+/* Host CSR (CSRGraph). */
+typedef struct dlp_graph_view {
+  int numNodes;
+  int numEdges;
+  const int *rowPtr;   /* numNodes + 1 */
+  const int *colInd;   /* numEdges     */
+  const float *values; /* numEdges     */
+} dlp_graph_view;
+
+/* std::vector<BatchData> flattened: batch b owns [off[b], off[b+1]) of each
+ * list. */
+typedef struct dlp_batch_view {
+  int numBatches;
+  const int64_t *insertedOffsets; /* numBatches + 1 */
+  const int *insertedNodeIds;
+  const int64_t *groundTruthOffsets;
+  const int *groundTruthNodeIds;
+  const float *groundTruthLabels;
+  const int64_t *deletedOffsets;
+  const int *deletedNodeIds;
+  const int64_t *edgeOffsets;
+  const int *newEdgeSrc;
+  const int *newEdgeDst;
+  const float *newEdgeWeight;
+} dlp_batch_view;
+
+/* Replaces runDynLP (dynLP.h:69-74, dynLP.cu:175-587).  finalLabels: N
+ * floats; batchTimes / batchAccuracy: numBatches floats each (nullable).
+ * Batch times are device time per batch (HIP events). */
+int dlp_run_dynlp(const dlp_config *cfg, const dlp_graph_view *graph,
+                  const float *trueLabels, const dlp_batch_view *batches,
+                  float *finalLabels, float *batchTimes, float *batchAccuracy);
+
+/* ====================================================================== */
+/* 3. Context API: device-resident graph + batch stream                   */
+/* ====================================================================== */
+
+typedef struct dlp_context dlp_context;
+
+/* Per-batch counters of the last run of that batch. */
+typedef struct dlp_batch_stats {
+  int batch;
+  int numInserted;
+  int numDeleted;
+  int numNewEdges;
+  int numSubEdges;      /* directed G' entries                         */
+  int numComponents;
+  int numMarked;        /* |affected set| before the eligibility filter */
+  int frontier0;        /* first frontier size                           */
+  int iterations;       /* iterationsUsed                                */
+  int numTouched;       /* distinct vertices written by propagation      */
+  int64_t updates;      /* sum of frontier sizes over iterations         */
+  float timeMs;         /* device time of the batch                      */
+  float accuracy;
+} dlp_batch_stats;
+
+/* Tuning knobs of the device path (0 = built-in default). */
+typedef struct dlp_tuning {
+  int propGroupLanes;   /* lanes per frontier vertex: 4, 8, 16, 32 or 64 */
+  int propBlocks;       /* blocks of the propagation launch              */
+  int chunkIterations;  /* iterations enqueued between host checks       */
+} dlp_tuning;
+
+/* Uploads the full graph, true labels and every batch, allocates all device
+ * state once (no allocation while batches run). */
+int dlp_context_create(dlp_context **out, const dlp_config *cfg,
+                       const dlp_graph_view *graph, const float *trueLabels,
+                       const dlp_batch_view *batches, void *stream);
+int dlp_context_destroy(dlp_context *ctx);
+int dlp_context_set_tuning(dlp_context *ctx, const dlp_tuning *tuning);
+/* The tau actually used (auto tau resolved). */
+int dlp_context_tau(const dlp_context *ctx, float *tau);
+/* Device bytes held by the context. */
+int dlp_context_device_bytes(const dlp_context *ctx, int64_t *bytes);
+
+/* Back to the state before batch 0 (device fills only). */
+int dlp_context_reset(dlp_context *ctx);
+/* Runs batch b (must be the next batch).  Returns when the batch's work is
+ * enqueued; its propagation loop synchronises with the host between chunks of
+ * iterations. */
+int dlp_context_run_batch(dlp_context *ctx, int b);
+/* reset + every batch. */
+int dlp_context_run_all(dlp_context *ctx);
+int dlp_context_synchronize(dlp_context *ctx);
+/* Copies the current labels (N floats) to host. */
+int dlp_context_read_labels(dlp_context *ctx, float *labels);
+/* Stats of batch b from the most recent run (synchronises). */
+int dlp_context_batch_stats(dlp_context *ctx, int b, dlp_batch_stats *out);
+
+/* Snapshot of the last batch's intermediate state, for parity checks.
+ * All pointers are host buffers and nullable.  Sizes: *_N arrays hold N
+ * entries; csrRowPtr N+1; csrColInd/csrValues room for the full graph's
+ * numEdges; sub* room for 2 * numNewEdges of that batch; frontierSizes
+ * room for maxFrontierSizes entries. */
+typedef struct dlp_trace {
+  int *csrRowPtr;        /* device CSR during steps 1-3, standard layout   */
+  int *csrColInd;
+  float *csrValues;
+  int *affected_N;       /* 1 where marked by step 1 (before filtering)   */
+  int *subSrc;           /* G' in build order                             */
+  int *subDst;
+  float *subW;
+  int *componentId_N;
+  float *initLabels_N;   /* labels entering propagation                   */
+  int *frontierSizes;
+  int maxFrontierSizes;
+  /* outputs */
+  int numSubEdges;
+  int numComponents;
+  int numFrontierSizes;
+} dlp_trace;
+
+/* Enables (t != NULL) or disables tracing.  When enabled, each run_batch
+ * synchronises and fills t; the struct must outlive the batches traced. */
+int dlp_context_set_trace(dlp_context *ctx, dlp_trace *t);
+
+/* Propagation-kernel measurement: with profiling on, every propagation
+ * launch is bracketed by HIP events on the context's stream and the kernel
+ * counts the work it does. */
+typedef struct dlp_prop_profile {
+  int64_t launches;     /* propagation launches measured                    */
+  double kernelMs;      /* sum of their event-measured durations            */
+  int64_t edges;        /* row entries read by frontier vertices            */
+  int64_t changedEdges; /* sum over changed vertices of (row length + 1)    */
+  int64_t appended;     /* next-frontier queue appends                      */
+} dlp_prop_profile;
+
+/* Turns profiling on or off and clears the accumulated profile. */
+int dlp_context_set_profiling(dlp_context *ctx, int enable);
+/* Profile accumulated since profiling was last turned on (synchronises). */
+int dlp_context_prop_profile(dlp_context *ctx, dlp_prop_profile *out);
+
+/* ====================================================================== */
+/* 4. Synthetic inputs (reference generateSparseGraph.cu / generateBatches.cu) */
+/* ====================================================================== */
+
+typedef struct dlp_inputs dlp_inputs;
+
+/* generateSparseGraph (generateSparseGraph.h:42-44) + generateBatches
+ * (generateBatches.h:56-59), same RNG call sequence. */
+int dlp_inputs_generate(dlp_inputs **out, int totalNodes, int avgDegree,
+                        int numBatches, unsigned int seed,
+                        float groundTruthFraction, float deletedFraction);
+/* Views into the handle's memory (valid until dlp_inputs_free). */
+int dlp_inputs_graph(const dlp_inputs *in, dlp_graph_view *graph,
+                     const float **trueLabels);
+int dlp_inputs_batches(const dlp_inputs *in, dlp_batch_view *batches);
+void dlp_inputs_free(dlp_inputs *in);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DYNLP_C_H */

@@ -1,0 +1,11 @@
+"""MI355X-native DynLP batch update (gfx950 HIP kernels behind a C-ABI).
+
+The product is ``libdynlp.so`` (include/dynlp/dynlp_c.h) and the C++ drop-in
+API of include/dynlp/dynLP.h.  This package is the Python plumbing over that
+C-ABI used by the tests and the benchmark.
+"""
+from ._lib import DynLPError, device_count, load  # noqa: F401
+from .dynlp import Context, Inputs, TraceBuffers, generate, make_config, run_dynlp  # noqa: F401
+
+__all__ = ["Context", "Inputs", "TraceBuffers", "generate", "make_config", "run_dynlp",
+           "DynLPError", "device_count", "load"]

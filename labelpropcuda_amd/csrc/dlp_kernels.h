@@ -1,0 +1,142 @@
+/*
+ * dlp_kernels.h -- host launchers of the batch-update kernels (internal).
+ */
+#ifndef DLP_KERNELS_H
+#define DLP_KERNELS_H
+
+#include "dlp_internal.h"
+
+/* ------------------------------------------------------------------------ */
+/* propagation (dlp_propagate.hip)                                          */
+/* ------------------------------------------------------------------------ */
+struct PropArgs {
+  const int *rowBeg;  /* row start per vertex                                  */
+  const int *rowEnd;  /* standard CSR: rowPtr + 1 (used when rowLen == null)   */
+  const int *rowLen;  /* slack layout: active row length                       */
+  const int *colInd;
+  const float *values;
+  const int *status;  /* non-null: skip DELETED neighbours (general CSR input) */
+  float *lab0;        /* buffer read by even iterations                        */
+  float *lab1;
+  int *iterStamp;     /* INT_MAX = may not enter a frontier                    */
+  int *q0, *q1;       /* frontier queues                                       */
+  int *touched;       /* nullable: first write of a vertex in this batch       */
+  DevCounters *ctr;
+  int *iterSizes;     /* nullable: frontier size of every iteration            */
+  int iterSizesCap;
+  int profile;        /* accumulate the prof* work counters                    */
+  float delta;
+};
+
+/* one propagation iteration k (reads qcount[k%3]); groupLanes in
+ * {4,8,16,32,64} */
+int dlp_launch_propagate(const PropArgs &a, int k, int groupLanes, int blocks, hipStream_t s);
+
+/* ------------------------------------------------------------------------ */
+/* batch pipeline (dlp_batch.hip)                                           */
+/* ------------------------------------------------------------------------ */
+struct DState {
+  int N;
+  /* full graph */
+  const int *fRow;
+  const int *fCol;
+  const float *fVal;
+  /* active CSR, slack layout: row u occupies [fRow[u], fRow[u] + aLen[u]) */
+  int *aLen;
+  int *aCol;
+  float *aVal;
+  /* vertex state */
+  int *hStatus;   /* reference h_vertexStatus                               */
+  int *dStatus;   /* reference d_vertexStatus during the batch              */
+  int *gtClass;
+  const int *batchMap;
+  uint8_t *vflags;
+  float *lab0, *lab1;
+  const float *trueLab;
+  int *iterStamp, *markStamp, *compId, *par;
+  uint32_t *newBits;
+  int numWords;
+  /* per-batch scratch */
+  int *S;          /* inserted vertices, ascending                          */
+  int *cand;       /* marked vertices                                       */
+  int *q0, *q1, *touched;
+  int *subSrc, *subDst;
+  float *subW;
+  float *p0, *p1;  /* per-member weight to GT class 0 / 1                    */
+  int *keyA, *valA, *keyB, *valB, *keyC, *valC;
+  int *segStart, *segEnd;
+  float *red0, *red1; /* per-component tree scratch                          */
+  float *W0, *W1;
+  int *scratch;    /* scan / sort scratch                                   */
+  DevCounters *ctr;
+  DevBatchRecord *records;
+};
+
+/* a batch's inputs on the device */
+struct DBatch {
+  int b;
+  const int *ins;
+  int nIns;
+  const int *gtIds;
+  const float *gtLab;
+  int nGt;
+  const int *del;
+  int nDel;
+  const int *eSrc, *eDst;
+  const float *eW;
+  int nE;
+  /* previous batch's lists (flags to clear, device status to restore) */
+  const int *prevIns;
+  int nPrevIns;
+  const int *prevDel;
+  int nPrevDel;
+  const int *prevGt;
+  int nPrevGt;
+};
+
+int dlp_reset_state(const DState &d, hipStream_t s);
+int dlp_batch_begin(const DState &d, const DBatch &bt, hipStream_t s);
+int dlp_csr_update(const DState &d, const DBatch &bt, int markTag, hipStream_t s);
+int dlp_sorted_inserted(const DState &d, hipStream_t s);
+int dlp_sparsify(const DState &d, const DBatch &bt, float tau, int markTag, hipStream_t s);
+int dlp_components(const DState &d, const DBatch &bt, hipStream_t s);
+int dlp_clear_component_ids(const DState &d, const int *ins, int n, hipStream_t s);
+int dlp_label_init(const DState &d, const DBatch &bt, hipStream_t s);
+int dlp_labels_no_edges(const DState &d, const DBatch &bt, hipStream_t s);
+int dlp_frontier0(const DState &d, int bound, hipStream_t s);
+int dlp_batch_end(const DState &d, const DBatch &bt, hipStream_t s);
+/* active CSR -> standard CSR (trace export); rowPtr N+1 */
+int dlp_export_csr(const DState &d, int *rowPtr, int *colInd, float *values, hipStream_t s);
+/* affected flags (markStamp == tag) -> int[N] */
+int dlp_export_marks(const DState &d, int tag, int *out, hipStream_t s);
+
+/* ------------------------------------------------------------------------ */
+/* step API kernels on a standard CSR (dlp_steps.hip)                       */
+/* ------------------------------------------------------------------------ */
+int dlp_k_mark_deletions(int *rowPtr, int *colInd, int *status, int *isAffected,
+                         const int *deleted, int numDeleted, int numNodes, hipStream_t s);
+int dlp_k_sparsify(const int *src, const int *dst, const float *w, int numNewEdges,
+                   int numInserted, const int *isNew, float tau, int *isAffected, int *subSrc,
+                   int *subDst, float *subW, int *countDev, int *scratch, hipStream_t s);
+int dlp_k_fill_int(int *p, int value, long long n, hipStream_t s);
+int dlp_k_copy_float(float *dst, const float *src, long long n, hipStream_t s);
+/* ascending list of distinct inserted ids; its length goes to ctr->numNew */
+int dlp_k_sorted_list(const int *inserted, int numInserted, int numNodes, uint32_t *bits,
+                      int *sorted, DevCounters *ctr, int *scratch, hipStream_t s);
+/* `sorted` must hold numInserted distinct ids (checked by the caller) */
+int dlp_k_cc(const int *src, const int *dst, int numEdges, const int *inserted, int numInserted,
+             int *componentId, int numNodes, int *par, const int *sorted, DevCounters *ctr,
+             int *scratch, hipStream_t s);
+int dlp_k_label_init(const int *rowPtr, const int *colInd, const float *values, float *labels,
+                     const int *componentId, int numComponents, const int *inserted,
+                     int numInserted, const int *gtClass, const int *sorted, float *p0,
+                     float *p1, int *keyA, int *valA, int *keyB, int *valB, int *keyC, int *valC,
+                     int *segStart, int *segEnd, float *red0, float *red1, float *W0, float *W1,
+                     int *scratch, hipStream_t s);
+int dlp_k_prop_prepare(const int *isAffected, const int *status, const int *gtClass,
+                       int numNodes, int *iterStamp, int *q0, DevCounters *ctr, hipStream_t s);
+int dlp_k_prop_finish(float *labels, const int *gtClass, int numNodes, int *isAffected,
+                      const int *pending, DevCounters *ctr, int pendingIndex, int capped,
+                      hipStream_t s);
+
+#endif /* DLP_KERNELS_H */

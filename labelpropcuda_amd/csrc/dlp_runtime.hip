@@ -1,0 +1,851 @@
+/*
+ * dlp_runtime.hip -- device context, batch sequencing and the C-ABI of
+ * runDynLP / the context API.
+ *
+ * The reference's batch loop (dynLP.cu:284-558) rebuilds the active CSR on
+ * the host, uploads it and every per-vertex array each batch, and its steps
+ * malloc/free scratch and read flags back every iteration.  Here the full
+ * graph, the true labels and every batch's lists are uploaded once at
+ * context creation, all state and scratch is allocated once, and a batch is
+ * a sequence of kernels on one stream; the host only reads the frontier
+ * count between chunks of propagation iterations to know when to stop.
+ */
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "dlp_kernels.h"
+
+/* ------------------------------------------------------------------------ */
+/* errors                                                                   */
+/* ------------------------------------------------------------------------ */
+static thread_local std::string g_lastError;
+
+extern "C" void dlp_set_error_(const char *msg) { g_lastError = msg ? msg : ""; }
+
+void dlp_set_errorf_(const char *fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_lastError = buf;
+}
+
+int dlp_hip_fail_(hipError_t e, const char *what) {
+  dlp_set_errorf_("HIP error %d (%s) in %s", (int)e, hipGetErrorString(e), what);
+  return e == hipErrorOutOfMemory ? DLP_ERR_NOMEM : DLP_ERR_HIP;
+}
+
+extern "C" const char *dlp_last_error(void) { return g_lastError.c_str(); }
+extern "C" const char *dlp_version(void) { return "libdynlp 0.1 (gfx950)"; }
+
+extern "C" int dlp_device_count(int *count) {
+  if (!count) {
+    dlp_set_error_("dlp_device_count: NULL argument");
+    return DLP_ERR_INVALID;
+  }
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+  (void)hipGetLastError();
+  *count = n;
+  return DLP_OK;
+}
+
+static int need_device() {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+    (void)hipGetLastError();
+    dlp_set_error_("no GPU visible to HIP");
+    return DLP_ERR_NODEVICE;
+  }
+  return DLP_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* context                                                                  */
+/* ------------------------------------------------------------------------ */
+struct dlp_context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool ownStream = false;
+  dlp_config cfg{};
+  float tau = 0.5f;
+  int N = 0, B = 0;
+  long long nnzFull = 0;
+  std::vector<int64_t> insOff, gtOff, delOff, eOff;
+  std::vector<void *> allocs;
+  int64_t deviceBytes = 0;
+  DState d{};
+  int *dIns = nullptr, *dGtIds = nullptr, *dDel = nullptr, *dESrc = nullptr, *dEDst = nullptr;
+  float *dGtLab = nullptr, *dEW = nullptr;
+  int *iterSizes = nullptr;
+  int iterSizesCap = 0;
+  DevCounters *pinCtr = nullptr;
+  dlp_tuning tuning{8, 1024, 8};
+  int nextBatch = 0;
+  int markSeq = 0;
+  int lastCCBatch = -1;
+  std::vector<hipEvent_t> evStart, evEnd;
+  std::vector<DevBatchRecord> records;
+  std::vector<char> ran;
+  dlp_trace *trace = nullptr;
+  /* profiling: event pairs of the current batch's propagation launches */
+  int profiling = 0;
+  std::vector<hipEvent_t> profEv;
+  int profUsed = 0;
+  int64_t profLaunches = 0;
+  double profMs = 0.0;
+
+  template <class T>
+  int alloc(T **p, long long count) {
+    size_t bytes = (size_t)std::max<long long>(count, 1) * sizeof(T);
+    void *q = nullptr;
+    hipError_t e = hipMalloc(&q, bytes);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      dlp_set_errorf_("hipMalloc of %zu bytes failed (%s)", bytes, hipGetErrorString(e));
+      return DLP_ERR_NOMEM;
+    }
+    allocs.push_back(q);
+    deviceBytes += (int64_t)bytes;
+    *p = (T *)q;
+    return DLP_OK;
+  }
+  template <class T>
+  int upload(T **p, const T *host, long long count) {
+    int rc = alloc(p, count);
+    if (rc != DLP_OK) return rc;
+    if (count > 0) DLP_HIP(hipMemcpy(*p, host, (size_t)count * sizeof(T), hipMemcpyHostToDevice));
+    return DLP_OK;
+  }
+  void release() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    for (void *p : allocs) (void)hipFree(p);
+    allocs.clear();
+    for (hipEvent_t e : profEv) (void)hipEventDestroy(e);
+    profEv.clear();
+    for (hipEvent_t e : evStart) (void)hipEventDestroy(e);
+    for (hipEvent_t e : evEnd) (void)hipEventDestroy(e);
+    evStart.clear();
+    evEnd.clear();
+    if (pinCtr) (void)hipHostFree(pinCtr);
+    pinCtr = nullptr;
+    if (ownStream && stream) (void)hipStreamDestroy(stream);
+    stream = nullptr;
+  }
+};
+
+namespace {
+
+/* host-side validation of the inputs (the reference trusts them; bad ids
+ * would make device kernels index out of bounds) */
+int validate_inputs(const dlp_config *cfg, const dlp_graph_view *g, const float *trueLabels,
+                    const dlp_batch_view *bv) {
+  if (!cfg || !g || !bv || !trueLabels) {
+    dlp_set_error_("NULL config, graph, batches or trueLabels");
+    return DLP_ERR_INVALID;
+  }
+  const int N = g->numNodes;
+  if (N <= 0 || !g->rowPtr || (g->numEdges > 0 && (!g->colInd || !g->values))) {
+    dlp_set_error_("graph must have numNodes > 0 and non-NULL arrays");
+    return DLP_ERR_INVALID;
+  }
+  if (g->rowPtr[0] != 0 || g->rowPtr[N] != g->numEdges) {
+    dlp_set_error_("graph rowPtr must start at 0 and end at numEdges");
+    return DLP_ERR_INVALID;
+  }
+  for (int u = 0; u < N; ++u)
+    if (g->rowPtr[u + 1] < g->rowPtr[u]) {
+      dlp_set_errorf_("graph rowPtr decreases at row %d", u);
+      return DLP_ERR_INVALID;
+    }
+  for (int e = 0; e < g->numEdges; ++e)
+    if (g->colInd[e] < -1 || g->colInd[e] >= N) {
+      dlp_set_errorf_("graph colInd[%d] = %d out of range", e, g->colInd[e]);
+      return DLP_ERR_INVALID;
+    }
+  if (cfg->numBatches <= 0 || cfg->numBatches > bv->numBatches) {
+    dlp_set_errorf_("config.numBatches = %d but %d batches given", cfg->numBatches,
+                    bv->numBatches);
+    return DLP_ERR_INVALID;
+  }
+  const int B = cfg->numBatches;
+  std::vector<int> seen(N, -1), seenGt(N, -1);
+  auto checkIds = [&](const int64_t *off, const int *ids, const char *what, std::vector<int> *uniq,
+                      int b) -> int {
+    if (off[b + 1] < off[b] || (off[b + 1] > off[b] && !ids)) {
+      dlp_set_errorf_("batch %d: bad %s offsets", b, what);
+      return DLP_ERR_INVALID;
+    }
+    for (int64_t i = off[b]; i < off[b + 1]; ++i) {
+      int v = ids[i];
+      if (v < 0 || v >= N) {
+        dlp_set_errorf_("batch %d: %s id %d out of range", b, what, v);
+        return DLP_ERR_INVALID;
+      }
+      if (uniq) {
+        if ((*uniq)[v] == b) {
+          dlp_set_errorf_("batch %d: %s id %d appears twice", b, what, v);
+          return DLP_ERR_INVALID;
+        }
+        (*uniq)[v] = b;
+      }
+    }
+    return DLP_OK;
+  };
+  for (int b = 0; b < B; ++b) {
+    int rc;
+    if ((rc = checkIds(bv->insertedOffsets, bv->insertedNodeIds, "inserted", &seen, b)) != DLP_OK)
+      return rc;
+    if ((rc = checkIds(bv->groundTruthOffsets, bv->groundTruthNodeIds, "ground-truth", &seenGt,
+                       b)) != DLP_OK)
+      return rc;
+    if ((rc = checkIds(bv->deletedOffsets, bv->deletedNodeIds, "deleted", nullptr, b)) != DLP_OK)
+      return rc;
+    if ((rc = checkIds(bv->edgeOffsets, bv->newEdgeSrc, "edge source", nullptr, b)) != DLP_OK)
+      return rc;
+    if ((rc = checkIds(bv->edgeOffsets, bv->newEdgeDst, "edge destination", nullptr, b)) !=
+        DLP_OK)
+      return rc;
+    for (int64_t i = bv->edgeOffsets[b]; i < bv->edgeOffsets[b + 1]; ++i)
+      if (!std::isfinite(bv->newEdgeWeight[i])) {
+        dlp_set_errorf_("batch %d: edge weight %lld is not finite", b, (long long)i);
+        return DLP_ERR_INVALID;
+      }
+    for (int64_t i = bv->groundTruthOffsets[b]; i < bv->groundTruthOffsets[b + 1]; ++i)
+      if (std::isnan(bv->groundTruthLabels[i])) {
+        dlp_set_errorf_("batch %d: ground-truth label is NaN", b);
+        return DLP_ERR_INVALID;
+      }
+  }
+  if (cfg->maxIterations < 0) {
+    dlp_set_error_("maxIterations must be >= 0");
+    return DLP_ERR_INVALID;
+  }
+  return DLP_OK;
+}
+
+DBatch make_batch(const dlp_context *c, int b) {
+  DBatch bt{};
+  bt.b = b;
+  auto span = [](const std::vector<int64_t> &off, int b2, long long &n) {
+    n = off[b2 + 1] - off[b2];
+    return off[b2];
+  };
+  long long n;
+  long long o = span(c->insOff, b, n);
+  bt.ins = c->dIns + o;
+  bt.nIns = (int)n;
+  o = span(c->gtOff, b, n);
+  bt.gtIds = c->dGtIds + o;
+  bt.gtLab = c->dGtLab + o;
+  bt.nGt = (int)n;
+  o = span(c->delOff, b, n);
+  bt.del = c->dDel + o;
+  bt.nDel = (int)n;
+  o = span(c->eOff, b, n);
+  bt.eSrc = c->dESrc + o;
+  bt.eDst = c->dEDst + o;
+  bt.eW = c->dEW + o;
+  bt.nE = (int)n;
+  if (b > 0) {
+    o = span(c->insOff, b - 1, n);
+    bt.prevIns = c->dIns + o;
+    bt.nPrevIns = (int)n;
+    o = span(c->delOff, b - 1, n);
+    bt.prevDel = c->dDel + o;
+    bt.nPrevDel = (int)n;
+    o = span(c->gtOff, b - 1, n);
+    bt.prevGt = c->dGtIds + o;
+    bt.nPrevGt = (int)n;
+  }
+  return bt;
+}
+
+int read_counters(dlp_context *c) {
+  DLP_HIP(hipMemcpyAsync(c->pinCtr, c->d.ctr, sizeof(DevCounters), hipMemcpyDeviceToHost,
+                         c->stream));
+  DLP_HIP(hipStreamSynchronize(c->stream));
+  if (c->pinCtr->errorFlag) {
+    dlp_set_errorf_("device kernels reported a violated precondition (flags 0x%x)",
+                    c->pinCtr->errorFlag);
+    return DLP_ERR_INTERNAL;
+  }
+  return DLP_OK;
+}
+
+/* propagation loop: chunks of iterations, host reads the next frontier
+ * size after each chunk */
+int run_propagation(dlp_context *c, const PropArgs &a, int maxIter) {
+  int k = 0;
+  int chunk = 2;
+  const int maxChunk = std::max(1, c->tuning.chunkIterations);
+  while (k < maxIter) {
+    int n = std::min(std::min(chunk, maxChunk), maxIter - k);
+    for (int j = 0; j < n; ++j) {
+      if (c->profiling) {
+        while ((int)c->profEv.size() < c->profUsed + 2) {
+          hipEvent_t e;
+          DLP_HIP(hipEventCreate(&e));
+          c->profEv.push_back(e);
+        }
+        DLP_HIP(hipEventRecord(c->profEv[c->profUsed], c->stream));
+      }
+      int rc = dlp_launch_propagate(a, k + j, c->tuning.propGroupLanes, c->tuning.propBlocks,
+                                    c->stream);
+      if (rc != DLP_OK) return rc;
+      if (c->profiling) {
+        DLP_HIP(hipEventRecord(c->profEv[c->profUsed + 1], c->stream));
+        c->profUsed += 2;
+      }
+    }
+    k += n;
+    int rc = read_counters(c);
+    if (rc != DLP_OK) return rc;
+    if (c->pinCtr->qcount[k % 3] == 0) break;
+    chunk *= 2;
+  }
+  if (c->profiling) {  // the stream is idle here: collect this batch's launch times
+    for (int i = 0; i < c->profUsed; i += 2) {
+      float ms = 0.0f;
+      DLP_HIP(hipEventElapsedTime(&ms, c->profEv[i], c->profEv[i + 1]));
+      c->profMs += ms;
+      ++c->profLaunches;
+    }
+    c->profUsed = 0;
+  }
+  return DLP_OK;
+}
+
+int trace_pre_propagation(dlp_context *c, const DBatch &bt, int markTag) {
+  dlp_trace *t = c->trace;
+  const int N = c->N;
+  DLP_HIP(hipStreamSynchronize(c->stream));
+  std::vector<void *> tmp;
+  auto devbuf = [&](size_t bytes) -> void * {
+    void *p = nullptr;
+    if (hipMalloc(&p, std::max<size_t>(bytes, 4)) != hipSuccess) return nullptr;
+    tmp.push_back(p);
+    return p;
+  };
+  auto cleanup = [&]() {
+    for (void *p : tmp) (void)hipFree(p);
+  };
+  int rc = DLP_OK;
+  if (t->csrRowPtr && t->csrColInd && t->csrValues) {
+    int *rp = (int *)devbuf(sizeof(int) * (N + 1));
+    int *ci = (int *)devbuf(sizeof(int) * (size_t)std::max<long long>(c->nnzFull, 1));
+    float *vv = (float *)devbuf(sizeof(float) * (size_t)std::max<long long>(c->nnzFull, 1));
+    if (!rp || !ci || !vv) {
+      cleanup();
+      dlp_set_error_("trace: hipMalloc failed");
+      return DLP_ERR_NOMEM;
+    }
+    rc = dlp_export_csr(c->d, rp, ci, vv, c->stream);
+    if (rc == DLP_OK && hipStreamSynchronize(c->stream) == hipSuccess) {
+      (void)hipMemcpy(t->csrRowPtr, rp, sizeof(int) * (N + 1), hipMemcpyDeviceToHost);
+      int nnz = t->csrRowPtr[N];
+      (void)hipMemcpy(t->csrColInd, ci, sizeof(int) * nnz, hipMemcpyDeviceToHost);
+      (void)hipMemcpy(t->csrValues, vv, sizeof(float) * nnz, hipMemcpyDeviceToHost);
+    }
+  }
+  if (rc == DLP_OK && t->affected_N) {
+    int *m = (int *)devbuf(sizeof(int) * N);
+    rc = m ? dlp_export_marks(c->d, markTag, m, c->stream) : DLP_ERR_NOMEM;
+    if (rc == DLP_OK && hipStreamSynchronize(c->stream) == hipSuccess)
+      (void)hipMemcpy(t->affected_N, m, sizeof(int) * N, hipMemcpyDeviceToHost);
+  }
+  cleanup();
+  if (rc != DLP_OK) return rc;
+  DevCounters ctr;
+  DLP_HIP(hipMemcpy(&ctr, c->d.ctr, sizeof(DevCounters), hipMemcpyDeviceToHost));
+  const bool ccBranch = bt.nIns > 0 && bt.nE > 0;
+  t->numSubEdges = ccBranch ? 2 * ctr.subPairs : 0;
+  t->numComponents = ccBranch ? ctr.numComponents : bt.nIns;
+  if (t->numSubEdges > 0) {
+    if (t->subSrc)
+      DLP_HIP(hipMemcpy(t->subSrc, c->d.subSrc, sizeof(int) * t->numSubEdges,
+                        hipMemcpyDeviceToHost));
+    if (t->subDst)
+      DLP_HIP(hipMemcpy(t->subDst, c->d.subDst, sizeof(int) * t->numSubEdges,
+                        hipMemcpyDeviceToHost));
+    if (t->subW)
+      DLP_HIP(hipMemcpy(t->subW, c->d.subW, sizeof(float) * t->numSubEdges,
+                        hipMemcpyDeviceToHost));
+  }
+  if (t->componentId_N)
+    DLP_HIP(hipMemcpy(t->componentId_N, c->d.compId, sizeof(int) * N, hipMemcpyDeviceToHost));
+  if (t->initLabels_N)
+    DLP_HIP(hipMemcpy(t->initLabels_N, c->d.lab0, sizeof(float) * N, hipMemcpyDeviceToHost));
+  return DLP_OK;
+}
+
+}  // namespace
+
+extern "C" int dlp_context_create(dlp_context **out, const dlp_config *cfg,
+                                  const dlp_graph_view *g, const float *trueLabels,
+                                  const dlp_batch_view *bv, void *stream) {
+  if (!out) {
+    dlp_set_error_("dlp_context_create: out is NULL");
+    return DLP_ERR_INVALID;
+  }
+  *out = nullptr;
+  int rc = validate_inputs(cfg, g, trueLabels, bv);
+  if (rc != DLP_OK) return rc;
+  if ((rc = need_device()) != DLP_OK) return rc;
+
+  dlp_context *c = new dlp_context();
+  c->cfg = *cfg;
+  if (hipGetDevice(&c->device) != hipSuccess) {
+    delete c;
+    dlp_set_error_("hipGetDevice failed");
+    return DLP_ERR_HIP;
+  }
+  c->N = g->numNodes;
+  c->B = cfg->numBatches;
+  c->nnzFull = g->numEdges;
+  const int N = c->N, B = c->B;
+
+  /* tau: mean full-graph weight when not given (dynLP.cu:90-102, :186-190) */
+  float tau = cfg->tau;
+  if (tau < 0.0f) {
+    double sum = 0.0;
+    long long count = 0;
+    for (long long i = 0; i < g->numEdges; ++i)
+      if (g->colInd[i] >= 0) {
+        sum += g->values[i];
+        ++count;
+      }
+    tau = (g->numEdges == 0 || count == 0) ? 0.5f : (float)(sum / (double)count);
+  }
+  c->tau = tau;
+
+  /* batch offsets (host) and concatenated device lists */
+  auto offs = [&](const int64_t *o, std::vector<int64_t> &dst) {
+    dst.assign(o, o + B + 1);
+    for (int b = B; b >= 0; --b) dst[b] -= o[0];
+  };
+  offs(bv->insertedOffsets, c->insOff);
+  offs(bv->groundTruthOffsets, c->gtOff);
+  offs(bv->deletedOffsets, c->delOff);
+  offs(bv->edgeOffsets, c->eOff);
+  int maxIns = 0, maxE = 0;
+  for (int b = 0; b < B; ++b) {
+    maxIns = std::max<int>(maxIns, (int)(c->insOff[b + 1] - c->insOff[b]));
+    maxE = std::max<int>(maxE, (int)(c->eOff[b + 1] - c->eOff[b]));
+  }
+  std::vector<int> batchMap(N, -1);
+  for (int b = 0; b < B; ++b)
+    for (int64_t i = bv->insertedOffsets[b]; i < bv->insertedOffsets[b + 1]; ++i)
+      batchMap[bv->insertedNodeIds[i]] = b;
+
+  if (stream) {
+    c->stream = (hipStream_t)stream;
+  } else {
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+      delete c;
+      dlp_set_error_("hipStreamCreate failed");
+      return DLP_ERR_HIP;
+    }
+    c->ownStream = true;
+  }
+
+  DState &d = c->d;
+  d.N = N;
+  d.numWords = (N + 31) / 32;
+  const long long E = std::max<long long>(g->numEdges, 1);
+  int *fRow, *fCol, *bm;
+  float *fVal, *tl;
+#define TRY(x)            \
+  do {                    \
+    rc = (x);             \
+    if (rc != DLP_OK) {   \
+      c->release();       \
+      delete c;           \
+      return rc;          \
+    }                     \
+  } while (0)
+  TRY(c->upload(&fRow, g->rowPtr, N + 1));
+  TRY(c->upload(&fCol, g->colInd, g->numEdges));
+  TRY(c->upload(&fVal, g->values, g->numEdges));
+  TRY(c->upload(&tl, trueLabels, N));
+  TRY(c->upload(&bm, batchMap.data(), N));
+  d.fRow = fRow;
+  d.fCol = fCol;
+  d.fVal = fVal;
+  d.trueLab = tl;
+  d.batchMap = bm;
+  TRY(c->upload(&c->dIns, bv->insertedNodeIds + bv->insertedOffsets[0], c->insOff[B]));
+  TRY(c->upload(&c->dGtIds, bv->groundTruthNodeIds + bv->groundTruthOffsets[0], c->gtOff[B]));
+  TRY(c->upload(&c->dGtLab, bv->groundTruthLabels + bv->groundTruthOffsets[0], c->gtOff[B]));
+  TRY(c->upload(&c->dDel, bv->deletedNodeIds + bv->deletedOffsets[0], c->delOff[B]));
+  TRY(c->upload(&c->dESrc, bv->newEdgeSrc + bv->edgeOffsets[0], c->eOff[B]));
+  TRY(c->upload(&c->dEDst, bv->newEdgeDst + bv->edgeOffsets[0], c->eOff[B]));
+  TRY(c->upload(&c->dEW, bv->newEdgeWeight + bv->edgeOffsets[0], c->eOff[B]));
+
+  TRY(c->alloc(&d.aLen, N));
+  TRY(c->alloc(&d.aCol, E));
+  TRY(c->alloc(&d.aVal, E));
+  TRY(c->alloc(&d.hStatus, N));
+  TRY(c->alloc(&d.dStatus, N));
+  TRY(c->alloc(&d.gtClass, N));
+  TRY(c->alloc(&d.vflags, N));
+  TRY(c->alloc(&d.lab0, N));
+  TRY(c->alloc(&d.lab1, N));
+  TRY(c->alloc(&d.iterStamp, N));
+  TRY(c->alloc(&d.markStamp, N));
+  TRY(c->alloc(&d.compId, N));
+  TRY(c->alloc(&d.par, N));
+  TRY(c->alloc(&d.newBits, d.numWords));
+  TRY(c->alloc(&d.cand, N));
+  TRY(c->alloc(&d.q0, N));
+  TRY(c->alloc(&d.q1, N));
+  TRY(c->alloc(&d.touched, N));
+  const long long mi = std::max(maxIns, 1);
+  TRY(c->alloc(&d.S, mi));
+  TRY(c->alloc(&d.subSrc, 2LL * std::max(maxE, 1)));
+  TRY(c->alloc(&d.subDst, 2LL * std::max(maxE, 1)));
+  TRY(c->alloc(&d.subW, 2LL * std::max(maxE, 1)));
+  TRY(c->alloc(&d.p0, mi));
+  TRY(c->alloc(&d.p1, mi));
+  TRY(c->alloc(&d.keyA, mi));
+  TRY(c->alloc(&d.valA, mi));
+  TRY(c->alloc(&d.keyB, mi));
+  TRY(c->alloc(&d.valB, mi));
+  TRY(c->alloc(&d.keyC, mi));
+  TRY(c->alloc(&d.valC, mi));
+  TRY(c->alloc(&d.segStart, mi));
+  TRY(c->alloc(&d.segEnd, mi));
+  TRY(c->alloc(&d.red0, mi));
+  TRY(c->alloc(&d.red1, mi));
+  TRY(c->alloc(&d.W0, mi));
+  TRY(c->alloc(&d.W1, mi));
+  size_t scratch = std::max({dlp_scan_scratch_ints(d.numWords), dlp_scan_scratch_ints(maxE),
+                             dlp_scan_scratch_ints(mi), dlp_scan_scratch_ints(N),
+                             dlp_radix_scratch_ints(mi)});
+  TRY(c->alloc(&d.scratch, (long long)scratch));
+  TRY(c->alloc(&d.ctr, 1));
+  TRY(c->alloc(&d.records, B));
+  c->iterSizesCap = std::max(1, std::min(cfg->maxIterations, 1 << 20));
+  TRY(c->alloc(&c->iterSizes, c->iterSizesCap));
+#undef TRY
+  if (hipHostMalloc((void **)&c->pinCtr, sizeof(DevCounters), hipHostMallocDefault) !=
+      hipSuccess) {
+    c->release();
+    delete c;
+    dlp_set_error_("hipHostMalloc failed");
+    return DLP_ERR_NOMEM;
+  }
+  c->evStart.resize(B);
+  c->evEnd.resize(B);
+  for (int b = 0; b < B; ++b) {
+    if (hipEventCreate(&c->evStart[b]) != hipSuccess || hipEventCreate(&c->evEnd[b]) != hipSuccess) {
+      c->release();
+      delete c;
+      dlp_set_error_("hipEventCreate failed");
+      return DLP_ERR_HIP;
+    }
+  }
+  c->records.assign(B, DevBatchRecord{});
+  c->ran.assign(B, 0);
+  rc = dlp_context_reset(c);
+  if (rc != DLP_OK) {
+    c->release();
+    delete c;
+    return rc;
+  }
+  DLP_HIP(hipStreamSynchronize(c->stream));
+  *out = c;
+  return DLP_OK;
+}
+
+extern "C" int dlp_context_destroy(dlp_context *c) {
+  if (!c) return DLP_OK;
+  c->release();
+  delete c;
+  return DLP_OK;
+}
+
+extern "C" int dlp_context_set_tuning(dlp_context *c, const dlp_tuning *t) {
+  if (!c || !t) {
+    dlp_set_error_("dlp_context_set_tuning: NULL argument");
+    return DLP_ERR_INVALID;
+  }
+  dlp_tuning n = c->tuning;
+  if (t->propGroupLanes) {
+    int g = t->propGroupLanes;
+    if (g != 4 && g != 8 && g != 16 && g != 32 && g != 64) {
+      dlp_set_errorf_("propGroupLanes %d not in {4,8,16,32,64}", g);
+      return DLP_ERR_INVALID;
+    }
+    n.propGroupLanes = g;
+  }
+  if (t->propBlocks) {
+    if (t->propBlocks < 1 || t->propBlocks > 65536) {
+      dlp_set_error_("propBlocks must be in [1, 65536]");
+      return DLP_ERR_INVALID;
+    }
+    n.propBlocks = t->propBlocks;
+  }
+  if (t->chunkIterations) {
+    if (t->chunkIterations < 1 || t->chunkIterations > 4096) {
+      dlp_set_error_("chunkIterations must be in [1, 4096]");
+      return DLP_ERR_INVALID;
+    }
+    n.chunkIterations = t->chunkIterations;
+  }
+  c->tuning = n;
+  return DLP_OK;
+}
+
+extern "C" int dlp_context_tau(const dlp_context *c, float *tau) {
+  if (!c || !tau) {
+    dlp_set_error_("dlp_context_tau: NULL argument");
+    return DLP_ERR_INVALID;
+  }
+  *tau = c->tau;
+  return DLP_OK;
+}
+
+extern "C" int dlp_context_device_bytes(const dlp_context *c, int64_t *bytes) {
+  if (!c || !bytes) {
+    dlp_set_error_("dlp_context_device_bytes: NULL argument");
+    return DLP_ERR_INVALID;
+  }
+  *bytes = c->deviceBytes;
+  return DLP_OK;
+}
+
+extern "C" int dlp_context_reset(dlp_context *c) {
+  if (!c) {
+    dlp_set_error_("dlp_context_reset: NULL context");
+    return DLP_ERR_STATE;
+  }
+  int rc = dlp_reset_state(c->d, c->stream);
+  if (rc != DLP_OK) return rc;
+  c->nextBatch = 0;
+  c->lastCCBatch = -1;
+  std::fill(c->ran.begin(), c->ran.end(), 0);
+  return DLP_OK;
+}
+
+extern "C" int dlp_context_set_trace(dlp_context *c, dlp_trace *t) {
+  if (!c) {
+    dlp_set_error_("dlp_context_set_trace: NULL context");
+    return DLP_ERR_STATE;
+  }
+  c->trace = t;
+  return DLP_OK;
+}
+
+extern "C" int dlp_context_run_batch(dlp_context *c, int b) {
+  if (!c) {
+    dlp_set_error_("dlp_context_run_batch: NULL context");
+    return DLP_ERR_STATE;
+  }
+  if (b != c->nextBatch || b < 0 || b >= c->B) {
+    dlp_set_errorf_("batch %d requested, next batch is %d (reset to start over)", b,
+                    c->nextBatch);
+    return DLP_ERR_STATE;
+  }
+  hipStream_t s = c->stream;
+  const DState &d = c->d;
+  DBatch bt = make_batch(c, b);
+  int rc;
+  DLP_HIP(hipEventRecord(c->evStart[b], s));
+  if ((rc = dlp_batch_begin(d, bt, s)) != DLP_OK) return rc;
+  const int markTag = ++c->markSeq;
+  if ((rc = dlp_csr_update(d, bt, markTag, s)) != DLP_OK) return rc;
+  if (bt.nIns > 0 && bt.nE > 0) {
+    if (c->lastCCBatch >= 0) {
+      DBatch prev = make_batch(c, c->lastCCBatch);
+      if ((rc = dlp_clear_component_ids(d, prev.ins, prev.nIns, s)) != DLP_OK) return rc;
+    }
+    if ((rc = dlp_sorted_inserted(d, s)) != DLP_OK) return rc;
+    if ((rc = dlp_sparsify(d, bt, c->tau, markTag, s)) != DLP_OK) return rc;
+    if ((rc = dlp_components(d, bt, s)) != DLP_OK) return rc;
+    if ((rc = dlp_label_init(d, bt, s)) != DLP_OK) return rc;
+    c->lastCCBatch = b;
+  } else if (bt.nIns > 0) {
+    if ((rc = dlp_labels_no_edges(d, bt, s)) != DLP_OK) return rc;
+  }
+  if ((rc = dlp_frontier0(d, c->N, s)) != DLP_OK) return rc;
+  if (c->trace && (rc = trace_pre_propagation(c, bt, markTag)) != DLP_OK) return rc;
+
+  PropArgs a{};
+  a.rowBeg = d.fRow;
+  a.rowEnd = nullptr;
+  a.rowLen = d.aLen;
+  a.colInd = d.aCol;
+  a.values = d.aVal;
+  a.status = nullptr;  // every remaining entry points at a non-deleted vertex
+  a.lab0 = d.lab0;
+  a.lab1 = d.lab1;
+  a.iterStamp = d.iterStamp;
+  a.q0 = d.q0;
+  a.q1 = d.q1;
+  a.touched = d.touched;
+  a.ctr = d.ctr;
+  a.iterSizes = c->iterSizes;
+  a.iterSizesCap = c->iterSizesCap;
+  a.profile = c->profiling;
+  a.delta = c->cfg.delta;
+  if ((rc = run_propagation(c, a, c->cfg.maxIterations)) != DLP_OK) return rc;
+  if ((rc = dlp_batch_end(d, bt, s)) != DLP_OK) return rc;
+  DLP_HIP(hipEventRecord(c->evEnd[b], s));
+  c->ran[b] = 1;
+  c->nextBatch = b + 1;
+  if (c->trace) {
+    DLP_HIP(hipStreamSynchronize(s));
+    DevBatchRecord r;
+    DLP_HIP(hipMemcpy(&r, d.records + b, sizeof(r), hipMemcpyDeviceToHost));
+    c->trace->numFrontierSizes = r.iterations;
+    int nc = std::min(std::min(r.iterations, c->trace->maxFrontierSizes), c->iterSizesCap);
+    if (c->trace->frontierSizes && nc > 0)
+      DLP_HIP(hipMemcpy(c->trace->frontierSizes, c->iterSizes, sizeof(int) * nc,
+                        hipMemcpyDeviceToHost));
+  }
+  return DLP_OK;
+}
+
+extern "C" int dlp_context_set_profiling(dlp_context *c, int enable) {
+  if (!c) {
+    dlp_set_error_("dlp_context_set_profiling: NULL context");
+    return DLP_ERR_STATE;
+  }
+  DLP_HIP(hipStreamSynchronize(c->stream));
+  DevCounters ctr;
+  DLP_HIP(hipMemcpy(&ctr, c->d.ctr, sizeof(ctr), hipMemcpyDeviceToHost));
+  ctr.profEdges = ctr.profChangedEdges = ctr.profAppended = 0;
+  DLP_HIP(hipMemcpy(c->d.ctr, &ctr, sizeof(ctr), hipMemcpyHostToDevice));
+  c->profiling = enable ? 1 : 0;
+  c->profUsed = 0;
+  c->profLaunches = 0;
+  c->profMs = 0.0;
+  return DLP_OK;
+}
+
+extern "C" int dlp_context_prop_profile(dlp_context *c, dlp_prop_profile *out) {
+  if (!c || !out) {
+    dlp_set_error_("dlp_context_prop_profile: NULL argument");
+    return DLP_ERR_INVALID;
+  }
+  DLP_HIP(hipStreamSynchronize(c->stream));
+  DevCounters ctr;
+  DLP_HIP(hipMemcpy(&ctr, c->d.ctr, sizeof(ctr), hipMemcpyDeviceToHost));
+  out->launches = c->profLaunches;
+  out->kernelMs = c->profMs;
+  out->edges = (int64_t)ctr.profEdges;
+  out->changedEdges = (int64_t)ctr.profChangedEdges;
+  out->appended = (int64_t)ctr.profAppended;
+  return DLP_OK;
+}
+
+extern "C" int dlp_context_run_all(dlp_context *c) {
+  int rc = dlp_context_reset(c);
+  if (rc != DLP_OK) return rc;
+  for (int b = 0; b < c->B; ++b)
+    if ((rc = dlp_context_run_batch(c, b)) != DLP_OK) return rc;
+  return DLP_OK;
+}
+
+extern "C" int dlp_context_synchronize(dlp_context *c) {
+  if (!c) {
+    dlp_set_error_("dlp_context_synchronize: NULL context");
+    return DLP_ERR_STATE;
+  }
+  DLP_HIP(hipStreamSynchronize(c->stream));
+  return DLP_OK;
+}
+
+extern "C" int dlp_context_read_labels(dlp_context *c, float *labels) {
+  if (!c || !labels) {
+    dlp_set_error_("dlp_context_read_labels: NULL argument");
+    return DLP_ERR_INVALID;
+  }
+  DLP_HIP(hipMemcpyAsync(labels, c->d.lab0, sizeof(float) * c->N, hipMemcpyDeviceToHost,
+                         c->stream));
+  DLP_HIP(hipStreamSynchronize(c->stream));
+  return DLP_OK;
+}
+
+extern "C" int dlp_context_batch_stats(dlp_context *c, int b, dlp_batch_stats *out) {
+  if (!c || !out || b < 0 || b >= c->B) {
+    dlp_set_error_("dlp_context_batch_stats: bad argument");
+    return DLP_ERR_INVALID;
+  }
+  if (!c->ran[b]) {
+    dlp_set_errorf_("batch %d has not run since the last reset", b);
+    return DLP_ERR_STATE;
+  }
+  DLP_HIP(hipStreamSynchronize(c->stream));
+  DevBatchRecord r;
+  DLP_HIP(hipMemcpy(&r, c->d.records + b, sizeof(r), hipMemcpyDeviceToHost));
+  float ms = 0.0f;
+  DLP_HIP(hipEventElapsedTime(&ms, c->evStart[b], c->evEnd[b]));
+  DBatch bt = make_batch(c, b);
+  out->batch = b;
+  out->numInserted = bt.nIns;
+  out->numDeleted = bt.nDel;
+  out->numNewEdges = bt.nE;
+  const bool cc = bt.nIns > 0 && bt.nE > 0;
+  out->numSubEdges = cc ? r.numSub : 0;
+  out->numComponents = cc ? r.numComponents : bt.nIns;
+  out->numMarked = r.numCand;
+  out->frontier0 = r.frontier0;
+  out->iterations = r.iterations;
+  out->numTouched = r.touched;
+  out->updates = (int64_t)r.updates;
+  out->timeMs = ms;
+  out->accuracy = r.total > 0 ? (float)r.correct / (float)r.total : 1.0f;
+  return DLP_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* runDynLP                                                                 */
+/* ------------------------------------------------------------------------ */
+extern "C" int dlp_run_dynlp(const dlp_config *cfg, const dlp_graph_view *g,
+                             const float *trueLabels, const dlp_batch_view *bv,
+                             float *finalLabels, float *batchTimes, float *batchAccuracy) {
+  if (!finalLabels) {
+    dlp_set_error_("dlp_run_dynlp: finalLabels is NULL");
+    return DLP_ERR_INVALID;
+  }
+  dlp_context *c = nullptr;
+  int rc = dlp_context_create(&c, cfg, g, trueLabels, bv, nullptr);
+  if (rc != DLP_OK) return rc;
+  if (cfg->tau < 0.0f) std::printf("Auto tau (average edge weight): %g\n", c->tau);
+  for (int b = 0; b < c->B && rc == DLP_OK; ++b) {
+    rc = dlp_context_run_batch(c, b);
+    if (rc != DLP_OK) break;
+    dlp_batch_stats st;
+    rc = dlp_context_batch_stats(c, b, &st);
+    if (rc != DLP_OK) break;
+    if (batchTimes) batchTimes[b] = st.timeMs;
+    if (batchAccuracy) batchAccuracy[b] = st.accuracy;
+    if (cfg->verbose) {
+      DBatch bt = make_batch(c, b);
+      std::printf("--- Batch %d ---\n  Inserting %d vertices (%d GT), deleting %d\n", b, bt.nIns,
+                  bt.nGt, bt.nDel);
+      if (bt.nIns > 0 && bt.nE > 0) {
+        std::printf("  Step 1: %d connected components, %d subgraph edges (tau=%g)\n",
+                    st.numComponents, st.numSubEdges, c->tau);
+        std::printf("  Step 2: Labels initialized for %d new vertices\n", bt.nIns);
+      }
+      std::printf("  Step 3: Converged in %d iterations\n", st.iterations);
+      std::printf("  Accuracy: %g%%, Time: %g ms\n\n", st.accuracy * 100.0f, st.timeMs);
+      std::fflush(stdout);
+    }
+  }
+  if (rc == DLP_OK) rc = dlp_context_read_labels(c, finalLabels);
+  std::string err = g_lastError;
+  dlp_context_destroy(c);
+  g_lastError = err;
+  return rc;
+}

@@ -1,0 +1,342 @@
+/*
+ * dlp_step_api.hip -- C-ABI of the five step launchers (standard CSR input).
+ *
+ * Each entry point keeps the reference launcher's arguments and semantics
+ * (see dynlp_c.h).  Scratch comes from one workspace per device that grows
+ * on demand and is reused across calls, instead of the reference's
+ * malloc/free per call; calls that use it are serialised by a mutex and end
+ * with a stream synchronisation, so the workspace is free when they return.
+ */
+#include <algorithm>
+#include <map>
+#include <mutex>
+#include <vector>
+
+#include "dlp_kernels.h"
+
+namespace {
+
+struct Workspace {
+  std::vector<void *> bufs;
+  std::vector<size_t> sizes;
+  DevCounters *ctr = nullptr;
+  DevCounters *pin = nullptr;
+};
+
+std::mutex g_wsMutex;
+std::map<int, Workspace> g_ws;
+
+/* buffer `slot` of at least `bytes`, grown when too small */
+int ws_get(Workspace &w, int slot, size_t bytes, void **out) {
+  if ((int)w.bufs.size() <= slot) {
+    w.bufs.resize(slot + 1, nullptr);
+    w.sizes.resize(slot + 1, 0);
+  }
+  bytes = std::max<size_t>(bytes, 256);
+  if (w.sizes[slot] < bytes) {
+    if (w.bufs[slot]) (void)hipFree(w.bufs[slot]);
+    w.bufs[slot] = nullptr;
+    w.sizes[slot] = 0;
+    void *p = nullptr;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      dlp_set_errorf_("workspace hipMalloc of %zu bytes failed", bytes);
+      return DLP_ERR_NOMEM;
+    }
+    w.bufs[slot] = p;
+    w.sizes[slot] = bytes;
+  }
+  *out = w.bufs[slot];
+  return DLP_OK;
+}
+
+int ws_open(Workspace **out) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+    (void)hipGetLastError();
+    dlp_set_error_("no GPU visible to HIP");
+    return DLP_ERR_NODEVICE;
+  }
+  int dev = 0;
+  DLP_HIP(hipGetDevice(&dev));
+  Workspace &w = g_ws[dev];
+  if (!w.ctr) {
+    void *p = nullptr;
+    DLP_HIP(hipMalloc(&p, sizeof(DevCounters)));
+    w.ctr = (DevCounters *)p;
+    DLP_HIP(hipHostMalloc((void **)&w.pin, sizeof(DevCounters), hipHostMallocDefault));
+  }
+  *out = &w;
+  return DLP_OK;
+}
+
+/* counters <- zero, stampBase = 1 */
+int ws_reset_counters(Workspace &w, hipStream_t s) {
+  DevCounters z = {};
+  z.stampBase = 1;
+  *w.pin = z;
+  DLP_HIP(hipMemcpyAsync(w.ctr, w.pin, sizeof(DevCounters), hipMemcpyHostToDevice, s));
+  DLP_HIP(hipStreamSynchronize(s));
+  return DLP_OK;
+}
+
+int ws_read_counters(Workspace &w, hipStream_t s) {
+  DLP_HIP(hipMemcpyAsync(w.pin, w.ctr, sizeof(DevCounters), hipMemcpyDeviceToHost, s));
+  DLP_HIP(hipStreamSynchronize(s));
+  return DLP_OK;
+}
+
+/* ascending list of the inserted ids; fails unless they are distinct (the
+ * list is only complete then) */
+int sorted_distinct(Workspace &w, const int *inserted, int numInserted, int numNodes,
+                    uint32_t *bits, int *sorted, int *scratch, hipStream_t s, const char *who) {
+  int rc = dlp_k_sorted_list(inserted, numInserted, numNodes, bits, sorted, w.ctr, scratch, s);
+  if (rc != DLP_OK) return rc;
+  if ((rc = ws_read_counters(w, s)) != DLP_OK) return rc;
+  if (w.pin->numNew != numInserted) {
+    dlp_set_errorf_("%s: insertedNodes must be distinct", who);
+    return DLP_ERR_INVALID;
+  }
+  return DLP_OK;
+}
+
+#define WS_BUF(T, name, slot, count)                                    \
+  T *name = nullptr;                                                    \
+  do {                                                                  \
+    void *p_ = nullptr;                                                 \
+    int rc_ = ws_get(*w, slot, sizeof(T) * (size_t)(count), &p_);       \
+    if (rc_ != DLP_OK) return rc_;                                      \
+    name = (T *)p_;                                                     \
+  } while (0)
+
+}  // namespace
+
+extern "C" int dlp_mark_deletions(int *rowPtr, int *colInd, int *vertexStatus, int *isAffected,
+                                  const int *deletedNodes, int numDeleted, int numNodes,
+                                  void *stream) {
+  if (numDeleted == 0) return DLP_OK; /* sparsifySubgraph.cu:188 */
+  if (numDeleted < 0 || numNodes < 0 || !rowPtr || !colInd || !vertexStatus || !isAffected ||
+      !deletedNodes) {
+    dlp_set_error_("dlp_mark_deletions: bad argument");
+    return DLP_ERR_INVALID;
+  }
+  return dlp_k_mark_deletions(rowPtr, colInd, vertexStatus, isAffected, deletedNodes, numDeleted,
+                              numNodes, (hipStream_t)stream);
+}
+
+extern "C" int dlp_build_sparsified_subgraph(
+    int *rowPtr, int *colInd, float *values, int *isAffected, const int *vertexStatus,
+    const int *newEdgeSrc, const int *newEdgeDst, const float *newEdgeWeight, int numNewEdges,
+    const int *insertedNodes, int numInserted, const int *isNewVertex, float tau, int *subRowPtr,
+    int *subColInd, float *subValues, int *subEdgeSrc, int *numSubNodes, int *numSubEdges,
+    void *stream) {
+  (void)rowPtr;
+  (void)colInd;
+  (void)values;
+  (void)vertexStatus;
+  (void)insertedNodes;
+  (void)subRowPtr;
+  if (!numSubNodes || !numSubEdges) {
+    dlp_set_error_("dlp_build_sparsified_subgraph: NULL output count");
+    return DLP_ERR_INVALID;
+  }
+  if (numNewEdges == 0) { /* sparsifySubgraph.cu:213-217 */
+    *numSubNodes = numInserted;
+    *numSubEdges = 0;
+    return DLP_OK;
+  }
+  if (numNewEdges < 0 || !isAffected || !newEdgeSrc || !newEdgeDst || !newEdgeWeight ||
+      !isNewVertex || !subColInd || !subValues) {
+    dlp_set_error_("dlp_build_sparsified_subgraph: bad argument");
+    return DLP_ERR_INVALID;
+  }
+  *numSubNodes = numInserted;
+  std::lock_guard<std::mutex> lock(g_wsMutex);
+  Workspace *w;
+  int rc = ws_open(&w);
+  if (rc != DLP_OK) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  if ((rc = ws_reset_counters(*w, s)) != DLP_OK) return rc;
+  WS_BUF(int, scratch, 0, dlp_scan_scratch_ints(numNewEdges));
+  rc = dlp_k_sparsify(newEdgeSrc, newEdgeDst, newEdgeWeight, numNewEdges, numInserted,
+                      isNewVertex, tau, isAffected, subEdgeSrc, subColInd, subValues,
+                      &w->ctr->scanTotal, scratch, s);
+  if (rc != DLP_OK) return rc;
+  if ((rc = ws_read_counters(*w, s)) != DLP_OK) return rc;
+  *numSubEdges = 2 * w->pin->scanTotal;
+  return DLP_OK;
+}
+
+extern "C" int dlp_connected_components(const int *edgeSrc, const int *edgeDst, int numEdges,
+                                        const int *insertedNodes, int numInserted,
+                                        int *componentId, int *numComponents, int numNodes,
+                                        void *stream) {
+  if (!numComponents) {
+    dlp_set_error_("dlp_connected_components: NULL numComponents");
+    return DLP_ERR_INVALID;
+  }
+  if (numInserted == 0) { /* connectedComponents.cu:167-170 */
+    *numComponents = 0;
+    return DLP_OK;
+  }
+  if (numInserted < 0 || numEdges < 0 || numNodes <= 0 || !insertedNodes || !componentId ||
+      (numEdges > 0 && (!edgeSrc || !edgeDst))) {
+    dlp_set_error_("dlp_connected_components: bad argument");
+    return DLP_ERR_INVALID;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  if (numEdges == 0) { /* connectedComponents.cu:191-205 */
+    int rc = dlp_k_fill_int(componentId, -1, numNodes, s);
+    if (rc != DLP_OK) return rc;
+    *numComponents = numInserted;
+    DLP_HIP(hipStreamSynchronize(s));
+    return DLP_OK;
+  }
+  std::lock_guard<std::mutex> lock(g_wsMutex);
+  Workspace *w;
+  int rc = ws_open(&w);
+  if (rc != DLP_OK) return rc;
+  if ((rc = ws_reset_counters(*w, s)) != DLP_OK) return rc;
+  const int words = (numNodes + 31) / 32;
+  WS_BUF(int, par, 1, numNodes);
+  WS_BUF(uint32_t, bits, 2, words);
+  WS_BUF(int, sorted, 3, numInserted);
+  WS_BUF(int, scratch, 0,
+         std::max(dlp_scan_scratch_ints(words), dlp_scan_scratch_ints(numInserted)));
+  if ((rc = sorted_distinct(*w, insertedNodes, numInserted, numNodes, bits, sorted, scratch, s,
+                            "dlp_connected_components")) != DLP_OK)
+    return rc;
+  rc = dlp_k_cc(edgeSrc, edgeDst, numEdges, insertedNodes, numInserted, componentId, numNodes, par,
+                sorted, w->ctr, scratch, s);
+  if (rc != DLP_OK) return rc;
+  if ((rc = ws_read_counters(*w, s)) != DLP_OK) return rc;
+  if (w->pin->errorFlag) {
+    dlp_set_errorf_("dlp_connected_components: edge endpoint is not an inserted vertex "
+                    "(flags 0x%x)",
+                    w->pin->errorFlag);
+    return DLP_ERR_INVALID;
+  }
+  *numComponents = w->pin->numComponents;
+  return DLP_OK;
+}
+
+extern "C" int dlp_label_initialization(const int *rowPtr, const int *colInd,
+                                        const float *values, float *labels,
+                                        const int *vertexStatus, const int *componentId,
+                                        int numComponents, const int *insertedNodes,
+                                        int numInserted, const int *gtClass, int numNodes,
+                                        void *stream) {
+  (void)vertexStatus;
+  if (numInserted == 0) return DLP_OK; /* labelInitialization.cu:155 */
+  if (numInserted < 0 || numNodes <= 0 || numComponents < 0 || !rowPtr || !labels ||
+      !componentId || !insertedNodes || !gtClass) {
+    dlp_set_error_("dlp_label_initialization: bad argument");
+    return DLP_ERR_INVALID;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  std::lock_guard<std::mutex> lock(g_wsMutex);
+  Workspace *w;
+  int rc = ws_open(&w);
+  if (rc != DLP_OK) return rc;
+  if ((rc = ws_reset_counters(*w, s)) != DLP_OK) return rc;
+  const int words = (numNodes + 31) / 32;
+  const long long ni = numInserted, nc = std::max(numComponents, 1);
+  WS_BUF(uint32_t, bits, 2, words);
+  WS_BUF(int, sorted, 3, ni);
+  WS_BUF(float, p0, 4, ni);
+  WS_BUF(float, p1, 5, ni);
+  WS_BUF(int, keyA, 6, ni);
+  WS_BUF(int, valA, 7, ni);
+  WS_BUF(int, keyB, 8, ni);
+  WS_BUF(int, valB, 9, ni);
+  WS_BUF(int, keyC, 10, ni);
+  WS_BUF(int, valC, 11, ni);
+  WS_BUF(int, segStart, 12, nc);
+  WS_BUF(int, segEnd, 13, nc);
+  WS_BUF(float, red0, 14, ni);
+  WS_BUF(float, red1, 15, ni);
+  WS_BUF(float, W0, 16, nc);
+  WS_BUF(float, W1, 17, nc);
+  WS_BUF(int, scratch, 0,
+         std::max({dlp_scan_scratch_ints(words), dlp_scan_scratch_ints(ni),
+                   dlp_radix_scratch_ints(ni)}));
+  if ((rc = sorted_distinct(*w, insertedNodes, numInserted, numNodes, bits, sorted, scratch, s,
+                            "dlp_label_initialization")) != DLP_OK)
+    return rc;
+  rc = dlp_k_label_init(rowPtr, colInd, values, labels, componentId, numComponents, insertedNodes,
+                        numInserted, gtClass, sorted, p0, p1, keyA, valA, keyB, valB, keyC, valC,
+                        segStart, segEnd, red0, red1, W0, W1, scratch, s);
+  if (rc != DLP_OK) return rc;
+  DLP_HIP(hipStreamSynchronize(s));
+  return DLP_OK;
+}
+
+extern "C" int dlp_iterative_propagation(const int *rowPtr, const int *colInd,
+                                         const float *values, float *labels,
+                                         float *labelsBuffer, const int *vertexStatus,
+                                         const int *gtClass, int *isAffected, int numNodes,
+                                         float delta, int maxIterations, int *iterationsUsed,
+                                         void *stream) {
+  if (!iterationsUsed || !rowPtr || !labels || !labelsBuffer || !vertexStatus || !gtClass ||
+      !isAffected || numNodes < 0) {
+    dlp_set_error_("dlp_iterative_propagation: bad argument");
+    return DLP_ERR_INVALID;
+  }
+  *iterationsUsed = 0;
+  if (numNodes == 0) return DLP_OK;
+  hipStream_t s = (hipStream_t)stream;
+  std::lock_guard<std::mutex> lock(g_wsMutex);
+  Workspace *w;
+  int rc = ws_open(&w);
+  if (rc != DLP_OK) return rc;
+  if ((rc = ws_reset_counters(*w, s)) != DLP_OK) return rc;
+  WS_BUF(int, stamp, 1, numNodes);
+  WS_BUF(int, q0, 3, numNodes);
+  WS_BUF(int, q1, 18, numNodes);
+  /* both buffers equal on entry (iterativePropagation.cu:268-270) */
+  if ((rc = dlp_k_copy_float(labelsBuffer, labels, numNodes, s)) != DLP_OK) return rc;
+  if ((rc = dlp_k_prop_prepare(isAffected, vertexStatus, gtClass, numNodes, stamp, q0, w->ctr,
+                               s)) != DLP_OK)
+    return rc;
+  PropArgs a{};
+  a.rowBeg = rowPtr;
+  a.rowEnd = rowPtr + 1;
+  a.rowLen = nullptr;
+  a.colInd = colInd;
+  a.values = values;
+  a.status = vertexStatus;
+  a.lab0 = labels;
+  a.lab1 = labelsBuffer;
+  a.iterStamp = stamp;
+  a.q0 = q0;
+  a.q1 = q1;
+  a.touched = nullptr;
+  a.ctr = w->ctr;
+  a.iterSizes = nullptr;
+  a.iterSizesCap = 0;
+  a.delta = delta;
+  int k = 0, chunk = 2;
+  while (k < maxIterations) {
+    int n = std::min(std::min(chunk, 16), maxIterations - k);
+    for (int j = 0; j < n; ++j)
+      if ((rc = dlp_launch_propagate(a, k + j, 8, 1024, s)) != DLP_OK) return rc;
+    k += n;
+    if ((rc = ws_read_counters(*w, s)) != DLP_OK) return rc;
+    if (w->pin->qcount[k % 3] == 0) break;
+    chunk *= 2;
+  }
+  if ((rc = ws_read_counters(*w, s)) != DLP_OK) return rc;
+  const int m = w->pin->iterUsed;
+  *iterationsUsed = m;
+  /* last written buffer ends in labels (iterativePropagation.cu:330-334) */
+  if (m & 1) {
+    if ((rc = dlp_k_copy_float(labels, labelsBuffer, numNodes, s)) != DLP_OK) return rc;
+  }
+  const int capped = (m == maxIterations && m > 0 && w->pin->qcount[m % 3] > 0) ? 1 : 0;
+  rc = dlp_k_prop_finish(labels, gtClass, numNodes, m > 0 ? isAffected : nullptr,
+                         (m & 1) ? q1 : q0, w->ctr, m % 3, capped, s);
+  if (rc != DLP_OK) return rc;
+  DLP_HIP(hipStreamSynchronize(s));
+  return DLP_OK;
+}

@@ -1,0 +1,299 @@
+"""Python host layer over libdynlp's C-ABI (plumbing for tests and bench).
+
+Mirrors the reference's driver objects: ``Inputs`` holds what
+``generateSparseGraph`` + ``generateBatches`` produce (CSRGraph, trueLabels,
+std::vector<BatchData> flattened), ``run_dynlp`` is ``runDynLP``
+(dynLP.h:69-74), and ``Context`` is the device-resident form of the same batch
+loop with per-batch stats and an optional intermediate-state trace.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, load
+
+
+def _ptr(a: np.ndarray, ctype):
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+@dataclass
+class Inputs:
+    """Full graph, true labels and the flattened batch stream."""
+
+    rowPtr: np.ndarray
+    colInd: np.ndarray
+    values: np.ndarray
+    trueLabels: np.ndarray
+    insOff: np.ndarray
+    ins: np.ndarray
+    gtOff: np.ndarray
+    gtIds: np.ndarray
+    gtLab: np.ndarray
+    delOff: np.ndarray
+    dels: np.ndarray
+    eOff: np.ndarray
+    eSrc: np.ndarray
+    eDst: np.ndarray
+    eW: np.ndarray
+    _keep: list = field(default_factory=list, repr=False)
+
+    @property
+    def numNodes(self) -> int:
+        return int(self.rowPtr.shape[0] - 1)
+
+    @property
+    def numEdges(self) -> int:
+        return int(self.colInd.shape[0])
+
+    @property
+    def numBatches(self) -> int:
+        return int(self.insOff.shape[0] - 1)
+
+    def batch(self, b: int) -> dict:
+        s = slice
+        return {
+            "ins": self.ins[s(self.insOff[b], self.insOff[b + 1])],
+            "gtIds": self.gtIds[s(self.gtOff[b], self.gtOff[b + 1])],
+            "gtLab": self.gtLab[s(self.gtOff[b], self.gtOff[b + 1])],
+            "dels": self.dels[s(self.delOff[b], self.delOff[b + 1])],
+            "eSrc": self.eSrc[s(self.eOff[b], self.eOff[b + 1])],
+            "eDst": self.eDst[s(self.eOff[b], self.eOff[b + 1])],
+            "eW": self.eW[s(self.eOff[b], self.eOff[b + 1])],
+        }
+
+    def graph_view(self) -> _lib.GraphView:
+        g = _lib.GraphView()
+        g.numNodes = self.numNodes
+        g.numEdges = self.numEdges
+        g.rowPtr = _ptr(self.rowPtr, C.c_int)
+        g.colInd = _ptr(self.colInd, C.c_int)
+        g.values = _ptr(self.values, C.c_float)
+        return g
+
+    def batch_view(self) -> _lib.BatchView:
+        v = _lib.BatchView()
+        v.numBatches = self.numBatches
+        v.insertedOffsets = _ptr(self.insOff, C.c_int64)
+        v.insertedNodeIds = _ptr(self.ins, C.c_int)
+        v.groundTruthOffsets = _ptr(self.gtOff, C.c_int64)
+        v.groundTruthNodeIds = _ptr(self.gtIds, C.c_int)
+        v.groundTruthLabels = _ptr(self.gtLab, C.c_float)
+        v.deletedOffsets = _ptr(self.delOff, C.c_int64)
+        v.deletedNodeIds = _ptr(self.dels, C.c_int)
+        v.edgeOffsets = _ptr(self.eOff, C.c_int64)
+        v.newEdgeSrc = _ptr(self.eSrc, C.c_int)
+        v.newEdgeDst = _ptr(self.eDst, C.c_int)
+        v.newEdgeWeight = _ptr(self.eW, C.c_float)
+        return v
+
+
+def _copy(ptr, n, dtype):
+    if n == 0:
+        return np.zeros(0, dtype=dtype)
+    return np.ctypeslib.as_array(ptr, shape=(n,)).astype(dtype, copy=True)
+
+
+def generate(totalNodes: int, avgDegree: int, numBatches: int, seed: int = 42,
+             gtFraction: float = 0.01, deletedFraction: float = 0.09) -> Inputs:
+    """generateSparseGraph + generateBatches (same RNG sequence as the reference)."""
+    lib = load()
+    h = C.c_void_p()
+    check(lib.dlp_inputs_generate(C.byref(h), totalNodes, avgDegree, numBatches, seed,
+                                  gtFraction, deletedFraction))
+    try:
+        g = _lib.GraphView()
+        tl = _lib.f32p()
+        check(lib.dlp_inputs_graph(h, C.byref(g), C.byref(tl)))
+        bv = _lib.BatchView()
+        check(lib.dlp_inputs_batches(h, C.byref(bv)))
+        N, E, B = g.numNodes, g.numEdges, bv.numBatches
+        insOff = _copy(bv.insertedOffsets, B + 1, np.int64)
+        gtOff = _copy(bv.groundTruthOffsets, B + 1, np.int64)
+        delOff = _copy(bv.deletedOffsets, B + 1, np.int64)
+        eOff = _copy(bv.edgeOffsets, B + 1, np.int64)
+        return Inputs(
+            rowPtr=_copy(g.rowPtr, N + 1, np.int32),
+            colInd=_copy(g.colInd, E, np.int32),
+            values=_copy(g.values, E, np.float32),
+            trueLabels=_copy(tl, N, np.float32),
+            insOff=insOff, ins=_copy(bv.insertedNodeIds, int(insOff[-1]), np.int32),
+            gtOff=gtOff, gtIds=_copy(bv.groundTruthNodeIds, int(gtOff[-1]), np.int32),
+            gtLab=_copy(bv.groundTruthLabels, int(gtOff[-1]), np.float32),
+            delOff=delOff, dels=_copy(bv.deletedNodeIds, int(delOff[-1]), np.int32),
+            eOff=eOff, eSrc=_copy(bv.newEdgeSrc, int(eOff[-1]), np.int32),
+            eDst=_copy(bv.newEdgeDst, int(eOff[-1]), np.int32),
+            eW=_copy(bv.newEdgeWeight, int(eOff[-1]), np.float32),
+        )
+    finally:
+        lib.dlp_inputs_free(h)
+
+
+def make_config(inputs: Inputs, tau: float = -1.0, delta: float = 1e-4,
+                maxIterations: int = 100000, verbose: bool = False,
+                numBatches: int | None = None) -> _lib.Config:
+    cfg = _lib.Config()
+    load().dlp_config_default(C.byref(cfg))
+    cfg.totalNodes = inputs.numNodes
+    cfg.numBatches = inputs.numBatches if numBatches is None else numBatches
+    cfg.tau = tau
+    cfg.delta = delta
+    cfg.maxIterations = maxIterations
+    cfg.verbose = 1 if verbose else 0
+    cfg.validate = 0
+    return cfg
+
+
+def run_dynlp(inputs: Inputs, cfg: _lib.Config):
+    """runDynLP: returns (finalLabels, batchTimesMs, batchAccuracy)."""
+    lib = load()
+    N, B = inputs.numNodes, cfg.numBatches
+    labels = np.zeros(N, np.float32)
+    times = np.zeros(B, np.float32)
+    acc = np.zeros(B, np.float32)
+    g, bv = inputs.graph_view(), inputs.batch_view()
+    check(lib.dlp_run_dynlp(C.byref(cfg), C.byref(g), _ptr(inputs.trueLabels, C.c_float),
+                            C.byref(bv), _ptr(labels, C.c_float), _ptr(times, C.c_float),
+                            _ptr(acc, C.c_float)))
+    return labels, times, acc
+
+
+class TraceBuffers:
+    """Host buffers for dlp_trace (the intermediate state of one batch)."""
+
+    def __init__(self, inputs: Inputs, maxFrontierSizes: int = 1 << 16):
+        N, E = inputs.numNodes, inputs.numEdges
+        maxE = int(np.max(np.diff(inputs.eOff))) if inputs.numBatches else 0
+        self.rowPtr = np.zeros(N + 1, np.int32)
+        self.colInd = np.zeros(max(E, 1), np.int32)
+        self.values = np.zeros(max(E, 1), np.float32)
+        self.affected = np.zeros(N, np.int32)
+        self.subSrc = np.zeros(max(2 * maxE, 1), np.int32)
+        self.subDst = np.zeros(max(2 * maxE, 1), np.int32)
+        self.subW = np.zeros(max(2 * maxE, 1), np.float32)
+        self.compId = np.zeros(N, np.int32)
+        self.initLabels = np.zeros(N, np.float32)
+        self.frontier = np.zeros(maxFrontierSizes, np.int32)
+        t = _lib.Trace()
+        t.csrRowPtr = _ptr(self.rowPtr, C.c_int)
+        t.csrColInd = _ptr(self.colInd, C.c_int)
+        t.csrValues = _ptr(self.values, C.c_float)
+        t.affected_N = _ptr(self.affected, C.c_int)
+        t.subSrc = _ptr(self.subSrc, C.c_int)
+        t.subDst = _ptr(self.subDst, C.c_int)
+        t.subW = _ptr(self.subW, C.c_float)
+        t.componentId_N = _ptr(self.compId, C.c_int)
+        t.initLabels_N = _ptr(self.initLabels, C.c_float)
+        t.frontierSizes = _ptr(self.frontier, C.c_int)
+        t.maxFrontierSizes = maxFrontierSizes
+        self.t = t
+
+    def snapshot(self) -> dict:
+        nnz = int(self.rowPtr[-1])
+        ns = self.t.numSubEdges
+        nf = min(self.t.numFrontierSizes, self.frontier.shape[0])
+        return {
+            "rowPtr": self.rowPtr.copy(),
+            "colInd": self.colInd[:nnz].copy(),
+            "values": self.values[:nnz].copy(),
+            "affected": self.affected.copy(),
+            "subSrc": self.subSrc[:ns].copy(),
+            "subDst": self.subDst[:ns].copy(),
+            "subW": self.subW[:ns].copy(),
+            "numSubEdges": ns,
+            "numComponents": self.t.numComponents,
+            "compId": self.compId.copy(),
+            "initLabels": self.initLabels.copy(),
+            "frontierSizes": self.frontier[:nf].copy(),
+            "iterations": self.t.numFrontierSizes,
+        }
+
+
+class Context:
+    """Device-resident graph + batch stream (dlp_context_*)."""
+
+    def __init__(self, inputs: Inputs, cfg: _lib.Config):
+        self._lib = load()
+        self.inputs = inputs
+        self.cfg = cfg
+        self._h = C.c_void_p()
+        g, bv = inputs.graph_view(), inputs.batch_view()
+        check(self._lib.dlp_context_create(C.byref(self._h), C.byref(cfg), C.byref(g),
+                                           _ptr(inputs.trueLabels, C.c_float), C.byref(bv),
+                                           None))
+        self._trace = None
+
+    def close(self):
+        if self._h:
+            self._lib.dlp_context_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def tau(self) -> float:
+        t = C.c_float()
+        check(self._lib.dlp_context_tau(self._h, C.byref(t)))
+        return t.value
+
+    def device_bytes(self) -> int:
+        b = C.c_int64()
+        check(self._lib.dlp_context_device_bytes(self._h, C.byref(b)))
+        return b.value
+
+    def set_tuning(self, groupLanes=0, blocks=0, chunk=0):
+        t = _lib.Tuning(groupLanes, blocks, chunk)
+        check(self._lib.dlp_context_set_tuning(self._h, C.byref(t)))
+
+    def enable_trace(self, maxFrontierSizes: int = 1 << 16) -> TraceBuffers:
+        self._trace = TraceBuffers(self.inputs, maxFrontierSizes)
+        check(self._lib.dlp_context_set_trace(self._h, C.byref(self._trace.t)))
+        return self._trace
+
+    def disable_trace(self):
+        check(self._lib.dlp_context_set_trace(self._h, None))
+        self._trace = None
+
+    def set_profiling(self, enable: bool = True):
+        check(self._lib.dlp_context_set_profiling(self._h, 1 if enable else 0))
+
+    def prop_profile(self) -> dict:
+        p = _lib.PropProfile()
+        check(self._lib.dlp_context_prop_profile(self._h, C.byref(p)))
+        return {name: getattr(p, name) for name, _ in p._fields_}
+
+    def reset(self):
+        check(self._lib.dlp_context_reset(self._h))
+
+    def run_batch(self, b: int):
+        check(self._lib.dlp_context_run_batch(self._h, b))
+
+    def run_all(self):
+        check(self._lib.dlp_context_run_all(self._h))
+
+    def synchronize(self):
+        check(self._lib.dlp_context_synchronize(self._h))
+
+    def labels(self) -> np.ndarray:
+        out = np.zeros(self.inputs.numNodes, np.float32)
+        check(self._lib.dlp_context_read_labels(self._h, _ptr(out, C.c_float)))
+        return out
+
+    def stats(self, b: int) -> dict:
+        s = _lib.BatchStats()
+        check(self._lib.dlp_context_batch_stats(self._h, b, C.byref(s)))
+        return {name: getattr(s, name) for name, _ in s._fields_}

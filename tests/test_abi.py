@@ -1,0 +1,132 @@
+"""The C-ABI boundary without a GPU: the library loads, exports every entry
+point the headers declare, validates inputs before touching a device, and the
+C++ drop-in header compiles against it."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import labelpropcuda_amd as L
+from labelpropcuda_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "dynlp", "dynlp_c.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(dlp_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_every_declared_symbol_is_exported(lib):
+    names = declared_functions()
+    assert len(names) >= 25
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    assert set(names) <= set(_lib.SIGNATURES), "ctypes binding lags the header"
+
+
+def test_version_and_device_count(lib):
+    assert b"gfx950" in lib.dlp_version()
+    assert L.device_count() >= 0
+
+
+def test_context_rejects_invalid_inputs_before_device_use(c1_inputs):
+    import copy
+    bad = copy.deepcopy(c1_inputs)
+    bad.ins = bad.ins.copy()
+    bad.ins[bad.insOff[2]] = bad.ins[bad.insOff[2] + 1]  # duplicate inserted id
+    cfg = L.make_config(bad)
+    with pytest.raises(L.DynLPError) as e:
+        L.Context(bad, cfg)
+    assert e.value.code == _lib.DLP_ERR_INVALID and "twice" in str(e.value)
+
+    bad = copy.deepcopy(c1_inputs)
+    bad.eDst = bad.eDst.copy()
+    bad.eDst[5] = bad.numNodes + 3
+    with pytest.raises(L.DynLPError) as e:
+        L.Context(bad, L.make_config(bad))
+    assert "out of range" in str(e.value)
+
+    bad = copy.deepcopy(c1_inputs)
+    bad.eW = bad.eW.copy()
+    bad.eW[7] = np.nan
+    with pytest.raises(L.DynLPError) as e:
+        L.Context(bad, L.make_config(bad))
+    assert "finite" in str(e.value)
+
+    cfg = L.make_config(c1_inputs, numBatches=99)
+    with pytest.raises(L.DynLPError) as e:
+        L.Context(c1_inputs, cfg)
+    assert e.value.code == _lib.DLP_ERR_INVALID
+
+
+def test_context_without_gpu_reports_no_device(c1_inputs):
+    if L.device_count() > 0:
+        pytest.skip("a GPU is visible")
+    with pytest.raises(L.DynLPError) as e:
+        L.Context(c1_inputs, L.make_config(c1_inputs))
+    assert e.value.code == _lib.DLP_ERR_NODEVICE
+
+
+def test_step_api_early_returns_need_no_device(lib):
+    nsn, nse = C.c_int(-1), C.c_int(-1)
+    rc = lib.dlp_build_sparsified_subgraph(None, None, None, None, None, None, None, None, 0,
+                                           None, 17, None, C.c_float(0.5), None, None, None,
+                                           None, C.byref(nsn), C.byref(nse), None)
+    assert rc == 0 and nsn.value == 17 and nse.value == 0
+    nc = C.c_int(-1)
+    rc = lib.dlp_connected_components(None, None, 5, None, 0, None, C.byref(nc), 10, None)
+    assert rc == 0 and nc.value == 0
+    assert lib.dlp_mark_deletions(None, None, None, None, None, 0, 10, None) == 0
+    assert lib.dlp_label_initialization(None, None, None, None, None, None, 0, None, 0, None, 10,
+                                        None) == 0
+    rc = lib.dlp_iterative_propagation(None, None, None, None, None, None, None, None, 10,
+                                       C.c_float(1e-4), 5, None, None)
+    assert rc == _lib.DLP_ERR_INVALID
+
+
+def test_tuning_validation_needs_context(lib):
+    t = _lib.Tuning(7, 0, 0)
+    assert lib.dlp_context_set_tuning(None, C.byref(t)) == _lib.DLP_ERR_INVALID
+
+
+def test_cxx_dropin_header_compiles_and_links(tmp_path):
+    src = tmp_path / "dropin.cpp"
+    src.write_text(r'''
+#include "dynlp/dynLP.h"
+#include <cstdio>
+int main(int argc, char** argv) {
+  DynLPConfig cfg;
+  if (!parseArguments(argc, argv, cfg)) return 2;
+  CSRGraph g; std::vector<float> labels; std::vector<BatchData> batches;
+  if (!generateSparseGraph(cfg.totalNodes, cfg.avgDegree, cfg.seed, g, labels, "")) return 3;
+  if (!generateBatches(g, labels, cfg, batches)) return 4;
+  FlatBatches f = flattenBatches(batches);
+  dlp_batch_view v = f.view();
+  std::printf("%d %d %d %lld\n", g.numNodes, g.numEdges, v.numBatches,
+              (long long)v.insertedOffsets[v.numBatches]);
+  return 0;
+}
+''')
+    exe = tmp_path / "dropin"
+    lib_dir = os.path.join(ROOT, "labelpropcuda_amd")
+    subprocess.run(["g++", "-std=c++17", "-I", os.path.join(ROOT, "include"), str(src), "-o",
+                    str(exe), "-L", lib_dir, "-ldynlp", f"-Wl,-rpath,{lib_dir}"], check=True)
+    out = subprocess.run([str(exe), "--totalNodes", "1200", "--numBatches", "4", "--quiet"],
+                         check=True, capture_output=True, text=True).stdout
+    assert out.strip().splitlines()[-1] == "1200 6000 4 1200"
+
+
+def test_cli_help_and_flag_errors():
+    assert os.path.exists(_lib.CLI_PATH)
+    r = subprocess.run([_lib.CLI_PATH, "--help"], capture_output=True, text=True)
+    assert r.returncode == 1 and "--totalNodes" in r.stdout
+    r = subprocess.run([_lib.CLI_PATH, "--delta", "0"], capture_output=True, text=True)
+    assert r.returncode == 1 and "--delta must be positive" in r.stdout
+    r = subprocess.run([_lib.CLI_PATH, "--bogus"], capture_output=True, text=True)
+    assert r.returncode == 1 and "Unknown argument" in r.stdout
