@@ -25,6 +25,7 @@
 #ifndef DYNLP_C_H
 #define DYNLP_C_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -299,6 +300,15 @@ int dlp_inputs_graph(const dlp_inputs *in, dlp_graph_view *graph,
                      const float **trueLabels);
 int dlp_inputs_batches(const dlp_inputs *in, dlp_batch_view *batches);
 void dlp_inputs_free(dlp_inputs *in);
+
+/* ====================================================================== */
+/* 5. Device-memory plumbing for callers without a GPU runtime of their   */
+/*    own (the reference calls cudaMalloc/cudaMemcpy directly)            */
+/* ====================================================================== */
+int dlp_device_malloc(void **ptr, size_t bytes);
+int dlp_device_free(void *ptr);
+int dlp_copy_to_device(void *dst, const void *src, size_t bytes);
+int dlp_copy_to_host(void *dst, const void *src, size_t bytes);
 
 #ifdef __cplusplus
 }

@@ -175,6 +175,10 @@ SIGNATURES = {
     "dlp_inputs_graph": (C.c_int, [_VP, C.POINTER(GraphView), C.POINTER(f32p)]),
     "dlp_inputs_batches": (C.c_int, [_VP, C.POINTER(BatchView)]),
     "dlp_inputs_free": (None, [_VP]),
+    "dlp_device_malloc": (C.c_int, [C.POINTER(_VP), C.c_size_t]),
+    "dlp_device_free": (C.c_int, [_VP]),
+    "dlp_copy_to_device": (C.c_int, [_VP, _VP, C.c_size_t]),
+    "dlp_copy_to_host": (C.c_int, [_VP, _VP, C.c_size_t]),
 }
 
 _lib = None
@@ -214,3 +218,33 @@ def device_count() -> int:
     n = C.c_int(0)
     check(load().dlp_device_count(C.byref(n)))
     return n.value
+
+
+class DeviceArray:
+    """A device copy of a numpy array (dlp_device_malloc / dlp_copy_*), for
+    calling the raw-pointer step API."""
+
+    def __init__(self, host):
+        import numpy as np
+        self.host = np.ascontiguousarray(host)
+        self.ptr = C.c_void_p()
+        lib = load()
+        check(lib.dlp_device_malloc(C.byref(self.ptr), max(self.host.nbytes, 1)))
+        check(lib.dlp_copy_to_device(self.ptr, self.host.ctypes.data_as(C.c_void_p),
+                                     self.host.nbytes))
+
+    def numpy(self):
+        out = self.host.copy()
+        check(load().dlp_copy_to_host(out.ctypes.data_as(C.c_void_p), self.ptr, out.nbytes))
+        return out
+
+    def free(self):
+        if self.ptr:
+            load().dlp_device_free(self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass

@@ -287,7 +287,8 @@ int run_propagation(dlp_context *c, const PropArgs &a, int maxIter) {
   int chunk = 2;
   const int maxChunk = std::max(1, c->tuning.chunkIterations);
   while (k < maxIter) {
-    int n = std::min(std::min(chunk, maxChunk), maxIter - k);
+    chunk = std::min(chunk, maxChunk);  // chunk doubles below; keep it bounded
+    int n = std::min(chunk, maxIter - k);
     for (int j = 0; j < n; ++j) {
       if (c->profiling) {
         while ((int)c->profEv.size() < c->profUsed + 2) {
@@ -848,4 +849,44 @@ extern "C" int dlp_run_dynlp(const dlp_config *cfg, const dlp_graph_view *g,
   dlp_context_destroy(c);
   g_lastError = err;
   return rc;
+}
+
+/* ------------------------------------------------------------------------ */
+/* device-memory plumbing                                                   */
+/* ------------------------------------------------------------------------ */
+extern "C" int dlp_device_malloc(void **ptr, size_t bytes) {
+  if (!ptr) {
+    dlp_set_error_("dlp_device_malloc: NULL argument");
+    return DLP_ERR_INVALID;
+  }
+  int rc = need_device();
+  if (rc != DLP_OK) return rc;
+  DLP_HIP(hipMalloc(ptr, bytes ? bytes : 1));
+  return DLP_OK;
+}
+
+extern "C" int dlp_device_free(void *ptr) {
+  if (ptr) DLP_HIP(hipFree(ptr));
+  return DLP_OK;
+}
+
+extern "C" int dlp_copy_to_device(void *dst, const void *src, size_t bytes) {
+  if (bytes == 0) return DLP_OK;
+  if (!dst || !src) {
+    dlp_set_error_("dlp_copy_to_device: NULL argument");
+    return DLP_ERR_INVALID;
+  }
+  DLP_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  return DLP_OK;
+}
+
+extern "C" int dlp_copy_to_host(void *dst, const void *src, size_t bytes) {
+  if (bytes == 0) return DLP_OK;
+  if (!dst || !src) {
+    dlp_set_error_("dlp_copy_to_host: NULL argument");
+    return DLP_ERR_INVALID;
+  }
+  DLP_HIP(hipDeviceSynchronize());
+  DLP_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+  return DLP_OK;
 }

@@ -318,7 +318,8 @@ extern "C" int dlp_iterative_propagation(const int *rowPtr, const int *colInd,
   a.delta = delta;
   int k = 0, chunk = 2;
   while (k < maxIterations) {
-    int n = std::min(std::min(chunk, 16), maxIterations - k);
+    chunk = std::min(chunk, 16);  // chunk doubles below; keep it bounded
+    int n = std::min(chunk, maxIterations - k);
     for (int j = 0; j < n; ++j)
       if ((rc = dlp_launch_propagate(a, k + j, 8, 1024, s)) != DLP_OK) return rc;
     k += n;

@@ -219,28 +219,31 @@ from labelpropcuda_amd import _lib  # noqa: E402
 import ctypes as C  # noqa: E402
 import oracle.oracle as O  # noqa: E402
 
-_torch = None
+_alive = []  # device arrays stay allocated until the test ends
 
 
-@pytest.fixture(scope="module")
+@pytest.fixture()
 def torch_dev():
-    """torch provides device memory for the raw-pointer step API (plumbing)."""
-    global _torch
-    _torch = pytest.importorskip("torch")
-    return _torch
+    """Device memory for the raw-pointer step API comes from the library's
+    own plumbing entry points (dlp_device_malloc / dlp_copy_*)."""
+    yield True
+    for a in _alive:
+        a.free()
+    _alive.clear()
 
 
 def dev(a):
-    return _torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    d = _lib.DeviceArray(a)
+    _alive.append(d)
+    return d
 
 
 def ptr(t):
-    return C.c_void_p(t.data_ptr())
+    return t.ptr
 
 
 def host(t):
-    _torch.cuda.synchronize()
-    return t.cpu().numpy()
+    return t.numpy()
 
 
 def test_step_mark_deletions(gpu_available, torch_dev, lib, c1_inputs):
