@@ -29,13 +29,38 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 
 
-def propagation_bytes(prof: dict, updates: int) -> int:
-    """Algorithmic bytes of the propagation launches (4-byte ids, weights,
+def propagation_bytes(updates: int, edges: int, changed_edges: int, appended: int) -> int:
+    """Algorithmic bytes of propagation iterations (4-byte ids, weights,
     labels): per frontier vertex 20 (queue id 4, row start 4, row length 4,
     F_old[u] 4, F_new[u] 4) + 12 per row entry (column 4, weight 4,
     F_old[v] 4); per changed vertex 4 per stamp update of its row entries and
     itself; 4 per next-frontier append."""
-    return 20 * updates + 12 * prof["edges"] + 4 * prof["changedEdges"] + 4 * prof["appended"]
+    return 20 * updates + 12 * edges + 4 * changed_edges + 4 * appended
+
+
+def engine_rooflines(prof: dict, updates: int) -> dict:
+    """Per propagation kernel: launches, mean launch time and algorithmic
+    GB/s (the bytes above over the event-timed launch durations)."""
+    local_u = prof["localUpdates"]
+    eng = {
+        "k_propagate": (prof["launches"], prof["kernelMs"],
+                        propagation_bytes(updates - local_u, prof["edges"],
+                                          prof["changedEdges"], prof["appended"])),
+        "k_propagate_local": (prof["localLaunches"], prof["localKernelMs"],
+                              propagation_bytes(local_u, prof["localEdges"],
+                                                prof["localChangedEdges"],
+                                                prof["localAppended"])),
+    }
+    out = {}
+    for name, (n, ms, byts) in eng.items():
+        if n == 0:
+            continue
+        per_launch = byts / n
+        avg_s = ms / 1000.0 / n
+        out[name] = {"launches": n, "kernel_ms": ms, "avg_launch_us": avg_s * 1e6,
+                     "algorithmic_bytes_per_launch": per_launch,
+                     "achieved_gbs": per_launch / avg_s / 1e9 if avg_s > 0 else 0.0}
+    return out
 
 
 def cpu_baseline(inputs, budget_s: float = 40.0):
@@ -67,6 +92,8 @@ def main() -> int:
     ap.add_argument("--lanes", type=int, default=0, help="propagation lanes per vertex")
     ap.add_argument("--blocks", type=int, default=0, help="propagation blocks per launch")
     ap.add_argument("--chunk", type=int, default=0, help="iterations between host checks")
+    ap.add_argument("--local-threshold", type=int, default=0,
+                    help="frontier size for the LDS-resident kernel (<0 disables)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     args = ap.parse_args()
 
@@ -82,8 +109,9 @@ def main() -> int:
     inputs = L.generate(args.nodes, args.degree, args.batches, 42)
     cfg = L.make_config(inputs)
     ctx = L.Context(inputs, cfg)
-    if args.lanes or args.blocks or args.chunk:
-        ctx.set_tuning(args.lanes, args.blocks, args.chunk)
+    if args.lanes or args.blocks or args.chunk or args.local_threshold:
+        ctx.set_tuning(args.lanes, args.blocks, args.chunk, args.local_threshold)
+    tuning = ctx.tuning()
 
     for _ in range(args.warmup):
         ctx.run_all()
@@ -108,10 +136,13 @@ def main() -> int:
 
     ms_per_step = 1000.0 * wall / args.steps
     value = updates / (wall / args.steps)
-    launches = max(prof["launches"], 1)
-    bytes_per_launch = propagation_bytes(prof, updates * args.steps) / launches
-    avg_launch_s = prof["kernelMs"] / 1000.0 / launches
-    achieved = bytes_per_launch / avg_launch_s / 1e9
+    engines = engine_rooflines(prof, updates * args.steps)
+    dominant = max(engines, key=lambda k: engines[k]["kernel_ms"])
+    dom = engines[dominant]
+    for e in engines.values():
+        e["launches_per_step"] = e["launches"] / args.steps
+        e["kernel_ms_per_step"] = e["kernel_ms"] / args.steps
+    achieved = dom["achieved_gbs"]
 
     out = {
         "metric": "affected-vertex updates/s (plain 1-GPU loop)",
@@ -134,9 +165,11 @@ def main() -> int:
                              "p99": float(np.percentile(lat, 99))},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_propagate", "launches_per_step": prof["launches"] / args.steps,
-                     "avg_launch_us": avg_launch_s * 1e6,
-                     "algorithmic_bytes_per_launch": bytes_per_launch},
+                     "kernel": dominant,
+                     "avg_launch_us": dom["avg_launch_us"],
+                     "algorithmic_bytes_per_launch": dom["algorithmic_bytes_per_launch"]},
+        "propagation_kernels": engines,
+        "tuning": tuning,
     }
     if not args.no_cpu:
         v, done, t = cpu_baseline(inputs)

@@ -213,6 +213,11 @@ typedef struct dlp_tuning {
   int propGroupLanes;   /* lanes per frontier vertex: 4, 8, 16, 32 or 64 */
   int propBlocks;       /* blocks of the propagation launch              */
   int chunkIterations;  /* iterations enqueued between host checks       */
+  int localThreshold;   /* frontier size at or below which iterations run
+                           in the LDS-resident single-work-group kernel;
+                           < 0 disables it                               */
+  int localLdsBytes;    /* LDS the local kernel may use, at most the
+                           device's per-work-group limit (the default)   */
 } dlp_tuning;
 
 /* Uploads the full graph, true labels and every batch, allocates all device
@@ -221,7 +226,9 @@ int dlp_context_create(dlp_context **out, const dlp_config *cfg,
                        const dlp_graph_view *graph, const float *trueLabels,
                        const dlp_batch_view *batches, void *stream);
 int dlp_context_destroy(dlp_context *ctx);
+/* fields left 0 keep their current value */
 int dlp_context_set_tuning(dlp_context *ctx, const dlp_tuning *tuning);
+int dlp_context_get_tuning(const dlp_context *ctx, dlp_tuning *tuning);
 /* The tau actually used (auto tau resolved). */
 int dlp_context_tau(const dlp_context *ctx, float *tau);
 /* Device bytes held by the context. */
@@ -272,11 +279,19 @@ int dlp_context_set_trace(dlp_context *ctx, dlp_trace *t);
  * launch is bracketed by HIP events on the context's stream and the kernel
  * counts the work it does. */
 typedef struct dlp_prop_profile {
-  int64_t launches;     /* propagation launches measured                    */
+  /* multi-block kernel (one launch per iteration) */
+  int64_t launches;     /* launches measured                                */
   double kernelMs;      /* sum of their event-measured durations            */
   int64_t edges;        /* row entries read by frontier vertices            */
   int64_t changedEdges; /* sum over changed vertices of (row length + 1)    */
   int64_t appended;     /* next-frontier queue appends                      */
+  /* LDS-resident kernel (many iterations per launch) */
+  int64_t localLaunches;
+  double localKernelMs;
+  int64_t localUpdates; /* frontier vertices it processed                   */
+  int64_t localEdges;
+  int64_t localChangedEdges;
+  int64_t localAppended;
 } dlp_prop_profile;
 
 /* Turns profiling on or off and clears the accumulated profile. */

@@ -97,6 +97,8 @@ class Tuning(C.Structure):
         ("propGroupLanes", C.c_int),
         ("propBlocks", C.c_int),
         ("chunkIterations", C.c_int),
+        ("localThreshold", C.c_int),
+        ("localLdsBytes", C.c_int),
     ]
 
 
@@ -126,6 +128,12 @@ class PropProfile(C.Structure):
         ("edges", C.c_int64),
         ("changedEdges", C.c_int64),
         ("appended", C.c_int64),
+        ("localLaunches", C.c_int64),
+        ("localKernelMs", C.c_double),
+        ("localUpdates", C.c_int64),
+        ("localEdges", C.c_int64),
+        ("localChangedEdges", C.c_int64),
+        ("localAppended", C.c_int64),
     ]
 
 
@@ -159,6 +167,7 @@ SIGNATURES = {
          _VP]),
     "dlp_context_destroy": (C.c_int, [_VP]),
     "dlp_context_set_tuning": (C.c_int, [_VP, C.POINTER(Tuning)]),
+    "dlp_context_get_tuning": (C.c_int, [_VP, C.POINTER(Tuning)]),
     "dlp_context_tau": (C.c_int, [_VP, f32p]),
     "dlp_context_device_bytes": (C.c_int, [_VP, i64p]),
     "dlp_context_reset": (C.c_int, [_VP]),

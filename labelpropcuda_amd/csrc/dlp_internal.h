@@ -52,10 +52,27 @@ struct DevCounters {
   unsigned long long total;    /* accuracy denominator                        */
   int scanTotal;       /* scratch total of the last scan                      */
   int pad[3];
+  /* LDS-resident propagation (single work-group) */
+  int localGen;        /* generation of the last local build (never reset)   */
+  int coreCount;       /* vertices in the saved core list (reset per batch)  */
+  int nextIter;        /* iteration the local kernel stopped at              */
+  int localReason;     /* why it stopped (DLP_LOCAL_*)                        */
   /* propagation work counters, accumulated while profiling is on */
   unsigned long long profEdges;        /* row entries read by frontier vertices */
   unsigned long long profChangedEdges; /* (row length + 1) of changed vertices  */
   unsigned long long profAppended;     /* queue appends                         */
+  /* the same counters for iterations run by the LDS-resident kernel */
+  unsigned long long profLocalUpdates;
+  unsigned long long profLocalEdges;
+  unsigned long long profLocalChangedEdges;
+  unsigned long long profLocalAppended;
+};
+
+/* exit reasons of the LDS-resident propagation kernel */
+enum {
+  DLP_LOCAL_CONVERGED = 1, /* next frontier empty                            */
+  DLP_LOCAL_CAPPED = 2,    /* maxIterations reached                          */
+  DLP_LOCAL_TOO_BIG = 3    /* working set does not fit; continue multi-block */
 };
 
 /* per-batch record kept on the device, read back once per run */

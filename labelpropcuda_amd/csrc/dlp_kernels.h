@@ -4,6 +4,8 @@
 #ifndef DLP_KERNELS_H
 #define DLP_KERNELS_H
 
+#include <vector>
+
 #include "dlp_internal.h"
 
 /* ------------------------------------------------------------------------ */
@@ -31,6 +33,56 @@ struct PropArgs {
 /* one propagation iteration k (reads qcount[k%3]); groupLanes in
  * {4,8,16,32,64} */
 int dlp_launch_propagate(const PropArgs &a, int k, int groupLanes, int blocks, hipStream_t s);
+
+/* global buffers of the LDS-resident propagation kernel */
+struct LocalArgs {
+  int numNodes;          /* length of coreGen / coreIdx                        */
+  int *coreGen;          /* N: generation in which the vertex is in the core;
+                            zero-filled at allocation, with ctr->localGen = 0  */
+  int *coreIdx;          /* N: its core index in that generation               */
+  int *coreList;         /* capacity coreListCap: core of the last build       */
+  int coreListCap;
+  int *ovf;              /* capacity ovfCap: marked vertices outside the core  */
+  int ovfCap;            /* >= ldsWords / 4                                    */
+  int ldsWords;          /* dynamic LDS of the launch, in 4-byte words          */
+};
+/* allocation sizes of the LocalArgs buffers for ldsWords words of LDS */
+inline int dlp_local_list_cap(int ldsWords) { return ldsWords / 4 + 64; }
+
+/* Runs propagation iterations from k in one work-group with the frontier's
+ * rows and neighbour labels held in LDS, until the frontier empties, k
+ * reaches maxIter, or the working set outgrows LDS.  Leaves the same global
+ * state the multi-block kernel would (labels in both buffers, next queue and
+ * its count, stamps, touched list, iteration counters) and records the stop
+ * iteration and reason in ctr->nextIter / ctr->localReason. */
+int dlp_launch_propagate_local(const PropArgs &a, const LocalArgs &l, int k, int maxIter,
+                               hipStream_t s);
+/* dynamic LDS bytes the local kernel can use on the current device */
+int dlp_local_lds_bytes(int *bytes);
+
+/* HIP-event timing of propagation launches (optional) */
+struct PropTiming {
+  std::vector<hipEvent_t> ev; /* pairs, reused                          */
+  int used = 0;
+  int64_t launches = 0, localLaunches = 0;
+  double ms = 0.0, localMs = 0.0;
+  std::vector<char> isLocal;
+};
+
+struct PropLoopConfig {
+  int groupLanes;     /* multi-block kernel lanes per vertex                 */
+  int blocks;         /* multi-block kernel grid                             */
+  int maxChunk;       /* multi-block iterations between host checks          */
+  int localThreshold; /* frontier size at or below which the LDS kernel runs */
+};
+
+/* Runs propagation iterations 0 .. until the frontier empties or maxIter,
+ * starting from the queue and count already on the device (qcount[0]).
+ * `l` may be null (multi-block only).  `pin` is pinned host memory for the
+ * counter read-backs; `timing` may be null. */
+int dlp_run_propagation(const PropArgs &a, const LocalArgs *l, int maxIter,
+                        const PropLoopConfig &cfg, DevCounters *pin, hipStream_t s,
+                        PropTiming *timing);
 
 /* ------------------------------------------------------------------------ */
 /* batch pipeline (dlp_batch.hip)                                           */

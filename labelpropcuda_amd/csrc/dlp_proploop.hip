@@ -1,0 +1,119 @@
+/*
+ * dlp_proploop.hip -- host driver of the propagation loop.
+ *
+ * The reference launches three kernels per iteration and reads two flags
+ * back with a stream synchronisation each (iterativePropagation.cu:274-327).
+ * Here the loop state lives on the device and the host only decides, from
+ * the frontier size it reads back, which engine runs next:
+ *   - frontier <= localThreshold: the LDS-resident kernel, which runs many
+ *     iterations in one launch and returns when the frontier empties, the
+ *     iteration cap is reached, or the working set no longer fits;
+ *   - otherwise: a chunk of multi-block launches, one per iteration, whose
+ *     length doubles up to maxChunk while the frontier stays large.
+ */
+#include <algorithm>
+
+#include "dlp_kernels.h"
+
+namespace {
+
+int read_ctr(DevCounters *dev, DevCounters *pin, hipStream_t s) {
+  DLP_HIP(hipMemcpyAsync(pin, dev, sizeof(DevCounters), hipMemcpyDeviceToHost, s));
+  DLP_HIP(hipStreamSynchronize(s));
+  if (pin->errorFlag) {
+    dlp_set_errorf_("device kernels reported a violated precondition (flags 0x%x)",
+                    pin->errorFlag);
+    return DLP_ERR_INTERNAL;
+  }
+  return DLP_OK;
+}
+
+int begin_timed(PropTiming *t, hipStream_t s, bool local) {
+  if (!t) return DLP_OK;
+  while ((int)t->ev.size() < t->used + 2) {
+    hipEvent_t e;
+    DLP_HIP(hipEventCreate(&e));
+    t->ev.push_back(e);
+  }
+  if ((int)t->isLocal.size() < t->used / 2 + 1) t->isLocal.resize(t->used / 2 + 1);
+  t->isLocal[t->used / 2] = local ? 1 : 0;
+  DLP_HIP(hipEventRecord(t->ev[t->used], s));
+  return DLP_OK;
+}
+
+int end_timed(PropTiming *t, hipStream_t s) {
+  if (!t) return DLP_OK;
+  DLP_HIP(hipEventRecord(t->ev[t->used + 1], s));
+  t->used += 2;
+  return DLP_OK;
+}
+
+/* the stream is idle: fold the recorded launch times into the totals */
+int collect_timed(PropTiming *t) {
+  if (!t) return DLP_OK;
+  for (int i = 0; i < t->used; i += 2) {
+    float ms = 0.0f;
+    DLP_HIP(hipEventElapsedTime(&ms, t->ev[i], t->ev[i + 1]));
+    if (t->isLocal[i / 2]) {
+      t->localMs += ms;
+      ++t->localLaunches;
+    } else {
+      t->ms += ms;
+      ++t->launches;
+    }
+  }
+  t->used = 0;
+  return DLP_OK;
+}
+
+}  // namespace
+
+int dlp_run_propagation(const PropArgs &a, const LocalArgs *l, int maxIter,
+                        const PropLoopConfig &cfg, DevCounters *pin, hipStream_t s,
+                        PropTiming *timing) {
+  int rc;
+  int k = 0;
+  int chunk = 2;
+  int localAfter = 0; /* no local launch before this iteration (after an overflow) */
+  const int maxChunk = std::max(1, cfg.maxChunk);
+  if ((rc = read_ctr(a.ctr, pin, s)) != DLP_OK) return rc;
+  while (k < maxIter) {
+    const int n = pin->qcount[k % 3];
+    if (n == 0) break;
+    if (l && n <= cfg.localThreshold && k >= localAfter) {
+      if (pin->localGen > (1 << 30)) {  // restart the core generations
+        DLP_HIP(hipMemsetAsync(l->coreGen, 0, (size_t)l->numNodes * sizeof(int), s));
+        DLP_HIP(hipMemsetAsync(&a.ctr->localGen, 0, sizeof(int), s));
+      }
+      if ((rc = begin_timed(timing, s, true)) != DLP_OK) return rc;
+      if ((rc = dlp_launch_propagate_local(a, *l, k, maxIter, s)) != DLP_OK) return rc;
+      if ((rc = end_timed(timing, s)) != DLP_OK) return rc;
+      if ((rc = read_ctr(a.ctr, pin, s)) != DLP_OK) return rc;
+      const int next = pin->nextIter;
+      if (next < k || next > maxIter) {
+        dlp_set_errorf_("local propagation returned iteration %d from %d", next, k);
+        return DLP_ERR_INTERNAL;
+      }
+      if (pin->localReason == DLP_LOCAL_TOO_BIG) {
+        localAfter = next + std::max(2, maxChunk);
+        chunk = 2;
+      }
+      k = next;
+      if (pin->localReason == DLP_LOCAL_CONVERGED || pin->localReason == DLP_LOCAL_CAPPED)
+        break;
+      continue;
+    }
+    chunk = std::min(chunk, maxChunk);
+    const int m = std::min(chunk, maxIter - k);
+    for (int j = 0; j < m; ++j) {
+      if ((rc = begin_timed(timing, s, false)) != DLP_OK) return rc;
+      if ((rc = dlp_launch_propagate(a, k + j, cfg.groupLanes, cfg.blocks, s)) != DLP_OK)
+        return rc;
+      if ((rc = end_timed(timing, s)) != DLP_OK) return rc;
+    }
+    k += m;
+    if ((rc = read_ctr(a.ctr, pin, s)) != DLP_OK) return rc;
+    chunk *= 2;
+  }
+  return collect_timed(timing);
+}

@@ -1,0 +1,97 @@
+/*
+ * dlp_rowsum.h -- the propagation row sum, bit-identical to the reference's
+ * 256-thread block reduction (iterativePropagation.cu:126-150).
+ *
+ * The reference block sums a row of length L like this: thread t (0..255)
+ * accumulates row positions t, t+256, ... in order into slot t (positions
+ * that are soft-deleted or point at a deleted vertex add nothing), then
+ * s[t] += s[t+stride] for stride = 128, 64, ..., 1.
+ *
+ * Here G lanes (a power of two dividing 64) cooperate on one row.  Slot
+ * t in [0,64) belongs to lane t % G as register j = t / G and holds
+ * (s[t] + s[t+128]) + (s[t+64] + s[t+192]) -- the reference's first two tree
+ * levels; strides 32 .. G are the in-register halving over j and strides
+ * G/2 .. 1 are cross-lane shuffles.  Each addition pairs the same two
+ * partial sums as the reference block, so the result is the same float.
+ *
+ * The caller supplies `pos(p, w, s)`: for row position p it sets the weight
+ * w and the product s = w * F[v] of a contributing entry, or w = s = 0.
+ * Adding +0.0f leaves every value that can arise here unchanged, so
+ * non-contributing positions may be added as zeros.
+ */
+#ifndef DLP_ROWSUM_H
+#define DLP_ROWSUM_H
+
+#include <hip/hip_runtime.h>
+
+namespace dlp {
+
+template <class Pos>
+__device__ __forceinline__ void slot_accumulate(const Pos &pos, int len, int x, float &w,
+                                                float &s) {
+  w = 0.0f;
+  s = 0.0f;
+  for (int p = x; p < len; p += 256) {  // positions x, x+256, ... in order
+    float pw, ps;
+    pos(p, pw, ps);
+    w = w + pw;
+    s = s + ps;
+  }
+}
+
+/* W and S of one row, valid in the group's lane 0 (gl == 0).  Every lane of
+ * the group must call it; `len` is the row length (group-uniform). */
+template <int G, class Pos>
+__device__ __forceinline__ void group_row_sums(int gl, int len, const Pos &pos, float &W,
+                                               float &S) {
+  constexpr int J = 64 / G;
+  float ws[J], ss[J];
+  if (len <= 64) {
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+      const int p = gl + G * j;
+      float w = 0.0f, s = 0.0f;
+      if (p < len) pos(p, w, s);
+      ws[j] = w;
+      ss[j] = s;
+    }
+  } else {  // general form: four slots per register (strides 128, 64)
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+      const int t = gl + G * j;
+      float w0, s0, w1, s1, w2, s2, w3, s3;
+      slot_accumulate(pos, len, t, w0, s0);
+      slot_accumulate(pos, len, t + 64, w1, s1);
+      slot_accumulate(pos, len, t + 128, w2, s2);
+      slot_accumulate(pos, len, t + 192, w3, s3);
+      ws[j] = (w0 + w2) + (w1 + w3);
+      ss[j] = (s0 + s2) + (s1 + s3);
+    }
+  }
+#pragma unroll
+  for (int h = J / 2; h >= 1; h >>= 1) {  // strides 32 .. G
+#pragma unroll
+    for (int j = 0; j < h; ++j) {
+      ws[j] = ws[j] + ws[j + h];
+      ss[j] = ss[j] + ss[j + h];
+    }
+  }
+  W = ws[0];
+  S = ss[0];
+#pragma unroll
+  for (int off = G / 2; off >= 1; off >>= 1) {  // strides G/2 .. 1
+    W = W + __shfl_down(W, off, G);
+    S = S + __shfl_down(S, off, G);
+  }
+}
+
+/* the reference's label update from the row sums (iterativePropagation.cu:
+ * 153-171): F = S / W if W > 1e-10f else F_old, clamped to [0,1] */
+__device__ __forceinline__ float new_label(float W, float S, float oldLabel) {
+  float nl = (W > 1e-10f) ? S / W : oldLabel;
+  return fminf(fmaxf(nl, 0.0f), 1.0f);
+}
+
+}  // namespace dlp
+
+#endif /* DLP_ROWSUM_H */

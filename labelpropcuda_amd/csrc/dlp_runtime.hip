@@ -87,7 +87,9 @@ struct dlp_context {
   int *iterSizes = nullptr;
   int iterSizesCap = 0;
   DevCounters *pinCtr = nullptr;
-  dlp_tuning tuning{8, 1024, 8};
+  dlp_tuning tuning{8, 1024, 8, 0, 0}; /* local fields set at creation */
+  LocalArgs local{};
+  int localMaxBytes = 0;
   int nextBatch = 0;
   int markSeq = 0;
   int lastCCBatch = -1;
@@ -95,12 +97,9 @@ struct dlp_context {
   std::vector<DevBatchRecord> records;
   std::vector<char> ran;
   dlp_trace *trace = nullptr;
-  /* profiling: event pairs of the current batch's propagation launches */
+  /* profiling: event-timed propagation launches */
   int profiling = 0;
-  std::vector<hipEvent_t> profEv;
-  int profUsed = 0;
-  int64_t profLaunches = 0;
-  double profMs = 0.0;
+  PropTiming timing;
 
   template <class T>
   int alloc(T **p, long long count) {
@@ -128,8 +127,8 @@ struct dlp_context {
     if (stream) (void)hipStreamSynchronize(stream);
     for (void *p : allocs) (void)hipFree(p);
     allocs.clear();
-    for (hipEvent_t e : profEv) (void)hipEventDestroy(e);
-    profEv.clear();
+    for (hipEvent_t e : timing.ev) (void)hipEventDestroy(e);
+    timing.ev.clear();
     for (hipEvent_t e : evStart) (void)hipEventDestroy(e);
     for (hipEvent_t e : evEnd) (void)hipEventDestroy(e);
     evStart.clear();
@@ -268,60 +267,16 @@ DBatch make_batch(const dlp_context *c, int b) {
   return bt;
 }
 
-int read_counters(dlp_context *c) {
-  DLP_HIP(hipMemcpyAsync(c->pinCtr, c->d.ctr, sizeof(DevCounters), hipMemcpyDeviceToHost,
-                         c->stream));
-  DLP_HIP(hipStreamSynchronize(c->stream));
-  if (c->pinCtr->errorFlag) {
-    dlp_set_errorf_("device kernels reported a violated precondition (flags 0x%x)",
-                    c->pinCtr->errorFlag);
-    return DLP_ERR_INTERNAL;
-  }
-  return DLP_OK;
-}
-
-/* propagation loop: chunks of iterations, host reads the next frontier
- * size after each chunk */
 int run_propagation(dlp_context *c, const PropArgs &a, int maxIter) {
-  int k = 0;
-  int chunk = 2;
-  const int maxChunk = std::max(1, c->tuning.chunkIterations);
-  while (k < maxIter) {
-    chunk = std::min(chunk, maxChunk);  // chunk doubles below; keep it bounded
-    int n = std::min(chunk, maxIter - k);
-    for (int j = 0; j < n; ++j) {
-      if (c->profiling) {
-        while ((int)c->profEv.size() < c->profUsed + 2) {
-          hipEvent_t e;
-          DLP_HIP(hipEventCreate(&e));
-          c->profEv.push_back(e);
-        }
-        DLP_HIP(hipEventRecord(c->profEv[c->profUsed], c->stream));
-      }
-      int rc = dlp_launch_propagate(a, k + j, c->tuning.propGroupLanes, c->tuning.propBlocks,
-                                    c->stream);
-      if (rc != DLP_OK) return rc;
-      if (c->profiling) {
-        DLP_HIP(hipEventRecord(c->profEv[c->profUsed + 1], c->stream));
-        c->profUsed += 2;
-      }
-    }
-    k += n;
-    int rc = read_counters(c);
-    if (rc != DLP_OK) return rc;
-    if (c->pinCtr->qcount[k % 3] == 0) break;
-    chunk *= 2;
-  }
-  if (c->profiling) {  // the stream is idle here: collect this batch's launch times
-    for (int i = 0; i < c->profUsed; i += 2) {
-      float ms = 0.0f;
-      DLP_HIP(hipEventElapsedTime(&ms, c->profEv[i], c->profEv[i + 1]));
-      c->profMs += ms;
-      ++c->profLaunches;
-    }
-    c->profUsed = 0;
-  }
-  return DLP_OK;
+  PropLoopConfig cfg;
+  cfg.groupLanes = c->tuning.propGroupLanes;
+  cfg.blocks = c->tuning.propBlocks;
+  cfg.maxChunk = c->tuning.chunkIterations;
+  cfg.localThreshold = c->tuning.localThreshold;
+  /* the local kernel encodes outside vertices as -2 - (2 * id + 1) */
+  const bool local = c->tuning.localThreshold > 0 && c->N < (1 << 30);
+  return dlp_run_propagation(a, local ? &c->local : nullptr, maxIter, cfg, c->pinCtr, c->stream,
+                             c->profiling ? &c->timing : nullptr);
 }
 
 int trace_pre_propagation(dlp_context *c, const DBatch &bt, int markTag) {
@@ -535,6 +490,25 @@ extern "C" int dlp_context_create(dlp_context **out, const dlp_config *cfg,
   TRY(c->alloc(&d.records, B));
   c->iterSizesCap = std::max(1, std::min(cfg->maxIterations, 1 << 20));
   TRY(c->alloc(&c->iterSizes, c->iterSizesCap));
+  /* LDS-resident propagation: LDS budget, core bookkeeping, default threshold
+   * = the frontier whose rows (mean degree) fill the LDS once */
+  {
+    int ldsBytes = 0;
+    TRY(dlp_local_lds_bytes(&ldsBytes));
+    LocalArgs &l = c->local;
+    l.ldsWords = ldsBytes / 4;
+    l.numNodes = N;
+    l.coreListCap = l.ovfCap = dlp_local_list_cap(l.ldsWords);
+    TRY(c->alloc(&l.coreGen, N));
+    TRY(c->alloc(&l.coreIdx, N));
+    TRY(c->alloc(&l.coreList, l.coreListCap));
+    TRY(c->alloc(&l.ovf, l.ovfCap));
+    DLP_HIP(hipMemsetAsync(l.coreGen, 0, (size_t)N * sizeof(int), c->stream));
+    DLP_HIP(hipMemsetAsync(d.ctr, 0, sizeof(DevCounters), c->stream));
+    const long long meanDeg = (g->numEdges + N - 1) / N;
+    c->tuning.localThreshold = (int)std::max<long long>(1, l.ldsWords / (8 + 4 * meanDeg));
+    c->tuning.localLdsBytes = c->localMaxBytes = ldsBytes;
+  }
 #undef TRY
   if (hipHostMalloc((void **)&c->pinCtr, sizeof(DevCounters), hipHostMallocDefault) !=
       hipSuccess) {
@@ -601,7 +575,25 @@ extern "C" int dlp_context_set_tuning(dlp_context *c, const dlp_tuning *t) {
     }
     n.chunkIterations = t->chunkIterations;
   }
+  if (t->localThreshold) n.localThreshold = t->localThreshold < 0 ? -1 : t->localThreshold;
+  if (t->localLdsBytes) {
+    if (t->localLdsBytes < 4096 || t->localLdsBytes > c->localMaxBytes) {
+      dlp_set_errorf_("localLdsBytes must be in [4096, %d]", c->localMaxBytes);
+      return DLP_ERR_INVALID;
+    }
+    n.localLdsBytes = t->localLdsBytes & ~255;
+  }
   c->tuning = n;
+  c->local.ldsWords = n.localLdsBytes / 4;
+  return DLP_OK;
+}
+
+extern "C" int dlp_context_get_tuning(const dlp_context *c, dlp_tuning *t) {
+  if (!c || !t) {
+    dlp_set_error_("dlp_context_get_tuning: NULL argument");
+    return DLP_ERR_INVALID;
+  }
+  *t = c->tuning;
   return DLP_OK;
 }
 
@@ -724,11 +716,13 @@ extern "C" int dlp_context_set_profiling(dlp_context *c, int enable) {
   DevCounters ctr;
   DLP_HIP(hipMemcpy(&ctr, c->d.ctr, sizeof(ctr), hipMemcpyDeviceToHost));
   ctr.profEdges = ctr.profChangedEdges = ctr.profAppended = 0;
+  ctr.profLocalUpdates = ctr.profLocalEdges = ctr.profLocalChangedEdges = 0;
+  ctr.profLocalAppended = 0;
   DLP_HIP(hipMemcpy(c->d.ctr, &ctr, sizeof(ctr), hipMemcpyHostToDevice));
   c->profiling = enable ? 1 : 0;
-  c->profUsed = 0;
-  c->profLaunches = 0;
-  c->profMs = 0.0;
+  c->timing.used = 0;
+  c->timing.launches = c->timing.localLaunches = 0;
+  c->timing.ms = c->timing.localMs = 0.0;
   return DLP_OK;
 }
 
@@ -740,11 +734,17 @@ extern "C" int dlp_context_prop_profile(dlp_context *c, dlp_prop_profile *out) {
   DLP_HIP(hipStreamSynchronize(c->stream));
   DevCounters ctr;
   DLP_HIP(hipMemcpy(&ctr, c->d.ctr, sizeof(ctr), hipMemcpyDeviceToHost));
-  out->launches = c->profLaunches;
-  out->kernelMs = c->profMs;
+  out->launches = c->timing.launches;
+  out->kernelMs = c->timing.ms;
   out->edges = (int64_t)ctr.profEdges;
   out->changedEdges = (int64_t)ctr.profChangedEdges;
   out->appended = (int64_t)ctr.profAppended;
+  out->localLaunches = c->timing.localLaunches;
+  out->localKernelMs = c->timing.localMs;
+  out->localUpdates = (int64_t)ctr.profLocalUpdates;
+  out->localEdges = (int64_t)ctr.profLocalEdges;
+  out->localChangedEdges = (int64_t)ctr.profLocalChangedEdges;
+  out->localAppended = (int64_t)ctr.profLocalAppended;
   return DLP_OK;
 }
 

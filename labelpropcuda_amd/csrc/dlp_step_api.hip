@@ -21,6 +21,11 @@ struct Workspace {
   std::vector<size_t> sizes;
   DevCounters *ctr = nullptr;
   DevCounters *pin = nullptr;
+  /* LDS-resident propagation: core generations must outlive counter resets */
+  int localGen = 0;
+  void *coreGenBuf = nullptr; /* the slot buffer localGen refers to */
+  size_t coreGenBytes = 0;
+  int ldsBytes = 0;
 };
 
 std::mutex g_wsMutex;
@@ -75,6 +80,7 @@ int ws_open(Workspace **out) {
 int ws_reset_counters(Workspace &w, hipStream_t s) {
   DevCounters z = {};
   z.stampBase = 1;
+  z.localGen = w.localGen;
   *w.pin = z;
   DLP_HIP(hipMemcpyAsync(w.ctr, w.pin, sizeof(DevCounters), hipMemcpyHostToDevice, s));
   DLP_HIP(hipStreamSynchronize(s));
@@ -315,19 +321,47 @@ extern "C" int dlp_iterative_propagation(const int *rowPtr, const int *colInd,
   a.ctr = w->ctr;
   a.iterSizes = nullptr;
   a.iterSizesCap = 0;
+  a.profile = 0;
   a.delta = delta;
-  int k = 0, chunk = 2;
-  while (k < maxIterations) {
-    chunk = std::min(chunk, 16);  // chunk doubles below; keep it bounded
-    int n = std::min(chunk, maxIterations - k);
-    for (int j = 0; j < n; ++j)
-      if ((rc = dlp_launch_propagate(a, k + j, 8, 1024, s)) != DLP_OK) return rc;
-    k += n;
-    if ((rc = ws_read_counters(*w, s)) != DLP_OK) return rc;
-    if (w->pin->qcount[k % 3] == 0) break;
-    chunk *= 2;
+
+  LocalArgs l{};
+  if (!w->ldsBytes && (rc = dlp_local_lds_bytes(&w->ldsBytes)) != DLP_OK) return rc;
+  l.ldsWords = w->ldsBytes / 4;
+  l.numNodes = numNodes;
+  l.coreListCap = l.ovfCap = dlp_local_list_cap(l.ldsWords);
+  WS_BUF(int, coreGen, 19, numNodes);
+  WS_BUF(int, coreIdx, 20, numNodes);
+  WS_BUF(int, coreList, 21, l.coreListCap);
+  WS_BUF(int, ovf, 22, l.ovfCap);
+  if (coreGen != w->coreGenBuf || w->sizes[19] != w->coreGenBytes) {
+    /* new or regrown buffer: restart the generations (the counters were
+     * already uploaded with the old localGen) */
+    DLP_HIP(hipMemsetAsync(coreGen, 0, w->sizes[19], s));
+    DLP_HIP(hipMemsetAsync(&w->ctr->localGen, 0, sizeof(int), s));
+    w->coreGenBuf = coreGen;
+    w->coreGenBytes = w->sizes[19];
+    w->localGen = 0;
   }
+  l.coreGen = coreGen;
+  l.coreIdx = coreIdx;
+  l.coreList = coreList;
+  l.ovf = ovf;
+  PropLoopConfig cfg;
+  cfg.groupLanes = 8;
+  cfg.blocks = 1024;
+  cfg.maxChunk = 16;
+  {
+    int nnz = 0;
+    DLP_HIP(hipMemcpyAsync(&nnz, rowPtr + numNodes, sizeof(int), hipMemcpyDeviceToHost, s));
+    DLP_HIP(hipStreamSynchronize(s));
+    const long long meanDeg = ((long long)std::max(nnz, 0) + numNodes - 1) / numNodes;
+    cfg.localThreshold = (int)std::max<long long>(1, l.ldsWords / (8 + 4 * meanDeg));
+  }
+  rc = dlp_run_propagation(a, numNodes < (1 << 30) ? &l : nullptr, maxIterations, cfg, w->pin, s,
+                           nullptr);
+  if (rc != DLP_OK) return rc;
   if ((rc = ws_read_counters(*w, s)) != DLP_OK) return rc;
+  w->localGen = w->pin->localGen;
   const int m = w->pin->iterUsed;
   *iterationsUsed = m;
   /* last written buffer ends in labels (iterativePropagation.cu:330-334) */

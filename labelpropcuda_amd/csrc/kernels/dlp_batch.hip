@@ -126,6 +126,11 @@ __global__ void k_reset(DState d) {
     z.profEdges = d.ctr->profEdges;  // work counters survive resets
     z.profChangedEdges = d.ctr->profChangedEdges;
     z.profAppended = d.ctr->profAppended;
+    z.profLocalUpdates = d.ctr->profLocalUpdates;
+    z.profLocalEdges = d.ctr->profLocalEdges;
+    z.profLocalChangedEdges = d.ctr->profLocalChangedEdges;
+    z.profLocalAppended = d.ctr->profLocalAppended;
+    z.localGen = d.ctr->localGen;  // the core generations stored per vertex stay valid
     *d.ctr = z;
   }
 }
@@ -146,6 +151,7 @@ __global__ void k_clear_prev(DState d, DBatch bt) {
     c->updates = 0;
     c->correct = 0;
     c->total = 0;
+    c->coreCount = 0;
   }
   const int n1 = bt.nPrevIns, n2 = n1 + bt.nPrevDel, n3 = n2 + bt.nPrevGt;
   if (i >= n3) return;

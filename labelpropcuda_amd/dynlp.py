@@ -255,9 +255,16 @@ class Context:
         check(self._lib.dlp_context_device_bytes(self._h, C.byref(b)))
         return b.value
 
-    def set_tuning(self, groupLanes=0, blocks=0, chunk=0):
-        t = _lib.Tuning(groupLanes, blocks, chunk)
+    def set_tuning(self, groupLanes=0, blocks=0, chunk=0, localThreshold=0, localLdsBytes=0):
+        """0 keeps a setting; localThreshold < 0 disables the LDS-resident
+        propagation kernel."""
+        t = _lib.Tuning(groupLanes, blocks, chunk, localThreshold, localLdsBytes)
         check(self._lib.dlp_context_set_tuning(self._h, C.byref(t)))
+
+    def tuning(self) -> dict:
+        t = _lib.Tuning()
+        check(self._lib.dlp_context_get_tuning(self._h, C.byref(t)))
+        return {name: getattr(t, name) for name, _ in t._fields_}
 
     def enable_trace(self, maxFrontierSizes: int = 1 << 16) -> TraceBuffers:
         self._trace = TraceBuffers(self.inputs, maxFrontierSizes)

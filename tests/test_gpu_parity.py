@@ -116,6 +116,41 @@ def test_work_shapes_give_identical_results(gpu_available, c1_inputs, lanes, blo
         assert_labels_equal(ctx.labels(), orc.labels(), f"G={lanes} blocks={blocks}")
 
 
+@pytest.mark.parametrize("threshold,ldsBytes", [
+    (-1, 0),          # multi-block kernel only
+    (1 << 30, 0),     # LDS-resident kernel whenever the working set fits
+    (64, 0),          # small frontiers only: many engine switches
+    (1 << 30, 4096),  # 1024 LDS words: core rebuilds, retries, fallbacks
+    (200, 16384),
+])
+def test_engine_mix_gives_identical_results(gpu_available, c1_inputs, threshold, ldsBytes):
+    """The multi-block and the LDS-resident propagation kernels, in any mix,
+    leave the oracle's labels, iteration counts and frontier sizes."""
+    inp = c1_inputs
+    cfg = L.make_config(inp, maxIterations=3000)
+    orc = Oracle(inp, maxIterations=3000)
+    with L.Context(inp, cfg) as ctx:
+        ctx.set_tuning(localThreshold=threshold, localLdsBytes=ldsBytes)
+        ctx.set_profiling(True)
+        tr = ctx.enable_trace(maxFrontierSizes=1 << 12)
+        ctx.reset()
+        for b in range(inp.numBatches):
+            ctx.run_batch(b)
+            orc.run_batch(b)
+            snap = tr.snapshot()
+            fs = orc.frontier_sizes()
+            assert snap["iterations"] == orc.stats()["iterations"], f"batch {b}: iterations"
+            n = min(len(fs), len(snap["frontierSizes"]))
+            assert np.array_equal(snap["frontierSizes"][:n], fs[:n]), f"batch {b}: frontiers"
+            assert_labels_equal(ctx.labels(), orc.labels(), f"batch {b}: labels")
+            assert ctx.stats(b)["updates"] == orc.stats()["updates"]
+        prof = ctx.prop_profile()
+    if threshold < 0:
+        assert prof["localLaunches"] == 0
+    else:
+        assert prof["localLaunches"] > 0
+
+
 def test_replay_is_deterministic(gpu_available, c1_inputs):
     with L.Context(c1_inputs, L.make_config(c1_inputs, maxIterations=5000)) as ctx:
         ctx.run_all()
