@@ -39,8 +39,10 @@ def propagation_bytes(updates: int, edges: int, changed_edges: int, appended: in
 
 
 def engine_rooflines(prof: dict, updates: int) -> dict:
-    """Per propagation kernel: launches, mean launch time and algorithmic
-    GB/s (the bytes above over the event-timed launch durations)."""
+    """Per propagation kernel and step: launches, event-timed kernel time,
+    mean launch time and algorithmic GB/s (the bytes above over that time).
+    Multi-block launches are timed per chunk of back-to-back launches, so
+    their mean includes the dispatch gaps inside a chunk."""
     local_u = prof["localUpdates"]
     eng = {
         "k_propagate": (prof["launches"], prof["kernelMs"],
@@ -57,7 +59,8 @@ def engine_rooflines(prof: dict, updates: int) -> dict:
             continue
         per_launch = byts / n
         avg_s = ms / 1000.0 / n
-        out[name] = {"launches": n, "kernel_ms": ms, "avg_launch_us": avg_s * 1e6,
+        out[name] = {"launches_per_step": n, "kernel_ms_per_step": ms,
+                     "avg_launch_us": avg_s * 1e6,
                      "algorithmic_bytes_per_launch": per_launch,
                      "achieved_gbs": per_launch / avg_s / 1e9 if avg_s > 0 else 0.0}
     return out
@@ -94,6 +97,8 @@ def main() -> int:
     ap.add_argument("--chunk", type=int, default=0, help="iterations between host checks")
     ap.add_argument("--local-threshold", type=int, default=0,
                     help="frontier size for the LDS-resident kernel (<0 disables)")
+    ap.add_argument("--local-shape", type=int, default=0,
+                    help="work-group shape of the LDS-resident kernel (1-based)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     args = ap.parse_args()
 
@@ -109,16 +114,16 @@ def main() -> int:
     inputs = L.generate(args.nodes, args.degree, args.batches, 42)
     cfg = L.make_config(inputs)
     ctx = L.Context(inputs, cfg)
-    if args.lanes or args.blocks or args.chunk or args.local_threshold:
-        ctx.set_tuning(args.lanes, args.blocks, args.chunk, args.local_threshold)
+    ctx.set_tuning(args.lanes, args.blocks, args.chunk, args.local_threshold,
+                   localShape=args.local_shape)
     tuning = ctx.tuning()
 
     for _ in range(args.warmup):
         ctx.run_all()
     ctx.synchronize()
 
-    ctx.set_profiling(True)
     lat, updates = [], None
+    ctx.set_profiling(1)  # events per LDS-kernel launch and per multi-block chunk
     ctx.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -131,17 +136,25 @@ def main() -> int:
         updates = u
     ctx.synchronize()
     wall = time.perf_counter() - t0
-    prof = ctx.prop_profile()
+    timed = ctx.prop_profile()
+
+    # the work counters (identical every step) from one more step, outside
+    # the timed region
+    ctx.set_profiling(2)
+    ctx.run_all()
+    ctx.synchronize()
+    work = ctx.prop_profile()
+    ctx.set_profiling(0)
     ctx.close()
+    prof = dict(work)
+    for key in ("launches", "kernelMs", "localLaunches", "localKernelMs"):
+        prof[key] = timed[key] / args.steps  # per step, from the timed region
 
     ms_per_step = 1000.0 * wall / args.steps
     value = updates / (wall / args.steps)
-    engines = engine_rooflines(prof, updates * args.steps)
-    dominant = max(engines, key=lambda k: engines[k]["kernel_ms"])
+    engines = engine_rooflines(prof, updates)
+    dominant = max(engines, key=lambda k: engines[k]["kernel_ms_per_step"])
     dom = engines[dominant]
-    for e in engines.values():
-        e["launches_per_step"] = e["launches"] / args.steps
-        e["kernel_ms_per_step"] = e["kernel_ms"] / args.steps
     achieved = dom["achieved_gbs"]
 
     out = {

@@ -218,6 +218,10 @@ typedef struct dlp_tuning {
                            < 0 disables it                               */
   int localLdsBytes;    /* LDS the local kernel may use, at most the
                            device's per-work-group limit (the default)   */
+  int localShape;       /* work-group shape of the local kernel, 1-based:
+                           1 = 1024 threads, one lane per row of <= 32
+                           entries; 2 = 1024/16; 3 = 512/32; 4 = 256/32;
+                           5 = 1024/8                                    */
 } dlp_tuning;
 
 /* Uploads the full graph, true labels and every batch, allocates all device
@@ -294,8 +298,12 @@ typedef struct dlp_prop_profile {
   int64_t localAppended;
 } dlp_prop_profile;
 
-/* Turns profiling on or off and clears the accumulated profile. */
-int dlp_context_set_profiling(dlp_context *ctx, int enable);
+/* Sets the profiling level and clears the accumulated profile:
+ * 0 off; 1 HIP-event timing of every LDS-resident launch and of every chunk
+ * of back-to-back multi-block launches (no work counters; the host does not
+ * record events between launches); 2 events around every launch plus the
+ * work counters (edges, changed edges, appends). */
+int dlp_context_set_profiling(dlp_context *ctx, int level);
 /* Profile accumulated since profiling was last turned on (synchronises). */
 int dlp_context_prop_profile(dlp_context *ctx, dlp_prop_profile *out);
 

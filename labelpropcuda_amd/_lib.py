@@ -99,6 +99,7 @@ class Tuning(C.Structure):
         ("chunkIterations", C.c_int),
         ("localThreshold", C.c_int),
         ("localLdsBytes", C.c_int),
+        ("localShape", C.c_int),
     ]
 
 

@@ -94,6 +94,20 @@ __device__ __forceinline__ int dlp_rank_below(unsigned long long mask) {
 
 __device__ __forceinline__ unsigned long long dlp_ballot(bool p) { return __ballot(p ? 1 : 0); }
 
+/* inclusive prefix sum over the 64 lanes of a wave, all lanes active: DPP
+ * row shifts by 1, 2, 4, 8 scan each row of 16 lanes, then row_bcast:15
+ * adds row 0's total to row 1 (and row 2's to row 3) and row_bcast:31 adds
+ * rows 0+1 to rows 2 and 3 */
+__device__ __forceinline__ int dlp_wave_inclusive_sum(int v) {
+  v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false);  // row_shr:1
+  v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false);  // row_shr:2
+  v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false);  // row_shr:4
+  v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false);  // row_shr:8
+  v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false);  // row_bcast:15
+  v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false);  // row_bcast:31
+  return v;
+}
+
 /* Wave-aggregated append: every lane of the wave that is active calls this
  * (converged); lanes with `pred` get a distinct slot of `counter`.  One
  * atomic per wave.  Returns the slot or -1. */

@@ -45,6 +45,7 @@ struct LocalArgs {
   int *ovf;              /* capacity ovfCap: marked vertices outside the core  */
   int ovfCap;            /* >= ldsWords / 4                                    */
   int ldsWords;          /* dynamic LDS of the launch, in 4-byte words          */
+  int shape;             /* work-group shape, [0, dlp_local_shape_count())      */
 };
 /* allocation sizes of the LocalArgs buffers for ldsWords words of LDS */
 inline int dlp_local_list_cap(int ldsWords) { return ldsWords / 4 + 64; }
@@ -59,6 +60,9 @@ int dlp_launch_propagate_local(const PropArgs &a, const LocalArgs &l, int k, int
                                hipStream_t s);
 /* dynamic LDS bytes the local kernel can use on the current device */
 int dlp_local_lds_bytes(int *bytes);
+/* number of work-group shapes of the local kernel (0 = threads 1024, one
+ * lane per row up to 32 entries) */
+int dlp_local_shape_count();
 
 /* HIP-event timing of propagation launches (optional) */
 struct PropTiming {
@@ -66,7 +70,10 @@ struct PropTiming {
   int used = 0;
   int64_t launches = 0, localLaunches = 0;
   double ms = 0.0, localMs = 0.0;
-  std::vector<char> isLocal;
+  std::vector<char> isLocal;  /* per pair: local kernel launch          */
+  std::vector<int> count;     /* per pair: launches it brackets         */
+  bool perChunk = false;      /* one pair per chunk of multi-block
+                                 launches instead of one per launch     */
 };
 
 struct PropLoopConfig {

@@ -28,15 +28,20 @@ int read_ctr(DevCounters *dev, DevCounters *pin, hipStream_t s) {
   return DLP_OK;
 }
 
-int begin_timed(PropTiming *t, hipStream_t s, bool local) {
+int begin_timed(PropTiming *t, hipStream_t s, bool local, int launches) {
   if (!t) return DLP_OK;
   while ((int)t->ev.size() < t->used + 2) {
     hipEvent_t e;
     DLP_HIP(hipEventCreate(&e));
     t->ev.push_back(e);
   }
-  if ((int)t->isLocal.size() < t->used / 2 + 1) t->isLocal.resize(t->used / 2 + 1);
-  t->isLocal[t->used / 2] = local ? 1 : 0;
+  const int pair = t->used / 2;
+  if ((int)t->isLocal.size() < pair + 1) {
+    t->isLocal.resize(pair + 1);
+    t->count.resize(pair + 1);
+  }
+  t->isLocal[pair] = local ? 1 : 0;
+  t->count[pair] = launches;
   DLP_HIP(hipEventRecord(t->ev[t->used], s));
   return DLP_OK;
 }
@@ -56,10 +61,10 @@ int collect_timed(PropTiming *t) {
     DLP_HIP(hipEventElapsedTime(&ms, t->ev[i], t->ev[i + 1]));
     if (t->isLocal[i / 2]) {
       t->localMs += ms;
-      ++t->localLaunches;
+      t->localLaunches += t->count[i / 2];
     } else {
       t->ms += ms;
-      ++t->launches;
+      t->launches += t->count[i / 2];
     }
   }
   t->used = 0;
@@ -85,7 +90,7 @@ int dlp_run_propagation(const PropArgs &a, const LocalArgs *l, int maxIter,
         DLP_HIP(hipMemsetAsync(l->coreGen, 0, (size_t)l->numNodes * sizeof(int), s));
         DLP_HIP(hipMemsetAsync(&a.ctr->localGen, 0, sizeof(int), s));
       }
-      if ((rc = begin_timed(timing, s, true)) != DLP_OK) return rc;
+      if ((rc = begin_timed(timing, s, true, 1)) != DLP_OK) return rc;
       if ((rc = dlp_launch_propagate_local(a, *l, k, maxIter, s)) != DLP_OK) return rc;
       if ((rc = end_timed(timing, s)) != DLP_OK) return rc;
       if ((rc = read_ctr(a.ctr, pin, s)) != DLP_OK) return rc;
@@ -105,12 +110,16 @@ int dlp_run_propagation(const PropArgs &a, const LocalArgs *l, int maxIter,
     }
     chunk = std::min(chunk, maxChunk);
     const int m = std::min(chunk, maxIter - k);
+    PropTiming *perLaunch = (timing && !timing->perChunk) ? timing : nullptr;
+    if (timing && timing->perChunk && (rc = begin_timed(timing, s, false, m)) != DLP_OK)
+      return rc;
     for (int j = 0; j < m; ++j) {
-      if ((rc = begin_timed(timing, s, false)) != DLP_OK) return rc;
+      if ((rc = begin_timed(perLaunch, s, false, 1)) != DLP_OK) return rc;
       if ((rc = dlp_launch_propagate(a, k + j, cfg.groupLanes, cfg.blocks, s)) != DLP_OK)
         return rc;
-      if ((rc = end_timed(timing, s)) != DLP_OK) return rc;
+      if ((rc = end_timed(perLaunch, s)) != DLP_OK) return rc;
     }
+    if (timing && timing->perChunk && (rc = end_timed(timing, s)) != DLP_OK) return rc;
     k += m;
     if ((rc = read_ctr(a.ctr, pin, s)) != DLP_OK) return rc;
     chunk *= 2;

@@ -85,6 +85,45 @@ __device__ __forceinline__ void group_row_sums(int gl, int len, const Pos &pos, 
   }
 }
 
+constexpr int bit_reverse(int x, int bits) {
+  int r = 0;
+  for (int b = 0; b < bits; ++b) r |= ((x >> b) & 1) << (bits - 1 - b);
+  return r;
+}
+
+constexpr int log2_exact(int x) { return x <= 1 ? 0 : 1 + log2_exact(x / 2); }
+
+/* W and S of a short row (len <= L, L a power of two <= 64) summed by one
+ * lane.  Slots len .. 63 of the reference block hold zeros, so its strides
+ * 128 .. L add zeros only, and strides L/2 .. 1 form a halving tree over
+ * slots 0..L-1.  That tree splits first on the lowest bit of the slot index,
+ * so it is the adjacent-pair tree over the slots in bit-reversed order,
+ * which a binary counter reduces in one pass with log2(L)+1 partial sums
+ * per quantity (float addition is commutative, so only the grouping
+ * matters). */
+template <int L, class Pos>
+__device__ __forceinline__ void lane_row_sums(int len, const Pos &pos, float &W, float &S) {
+  static_assert(L >= 1 && L <= 64 && (L & (L - 1)) == 0, "L must be a power of two <= 64");
+  constexpr int LOG = log2_exact(L);
+  float sw[LOG + 1], ss[LOG + 1];
+#pragma unroll
+  for (int i = 0; i < L; ++i) {
+    const int j = bit_reverse(i, LOG);
+    float w = 0.0f, s = 0.0f;
+    if (j < len) pos(j, w, s);
+    int b = 0;
+#pragma unroll
+    for (; (i >> b) & 1; ++b) {
+      w = sw[b] + w;
+      s = ss[b] + s;
+    }
+    sw[b] = w;
+    ss[b] = s;
+  }
+  W = sw[LOG];
+  S = ss[LOG];
+}
+
 /* the reference's label update from the row sums (iterativePropagation.cu:
  * 153-171): F = S / W if W > 1e-10f else F_old, clamped to [0,1] */
 __device__ __forceinline__ float new_label(float W, float S, float oldLabel) {

@@ -87,7 +87,7 @@ struct dlp_context {
   int *iterSizes = nullptr;
   int iterSizesCap = 0;
   DevCounters *pinCtr = nullptr;
-  dlp_tuning tuning{8, 1024, 8, 0, 0}; /* local fields set at creation */
+  dlp_tuning tuning{8, 512, 8, 0, 0, 1}; /* local threshold and LDS set at creation */
   LocalArgs local{};
   int localMaxBytes = 0;
   int nextBatch = 0;
@@ -583,8 +583,16 @@ extern "C" int dlp_context_set_tuning(dlp_context *c, const dlp_tuning *t) {
     }
     n.localLdsBytes = t->localLdsBytes & ~255;
   }
+  if (t->localShape) {
+    if (t->localShape < 1 || t->localShape > dlp_local_shape_count()) {
+      dlp_set_errorf_("localShape must be in [1, %d]", dlp_local_shape_count());
+      return DLP_ERR_INVALID;
+    }
+    n.localShape = t->localShape;
+  }
   c->tuning = n;
   c->local.ldsWords = n.localLdsBytes / 4;
+  c->local.shape = n.localShape - 1;
   return DLP_OK;
 }
 
@@ -687,7 +695,7 @@ extern "C" int dlp_context_run_batch(dlp_context *c, int b) {
   a.ctr = d.ctr;
   a.iterSizes = c->iterSizes;
   a.iterSizesCap = c->iterSizesCap;
-  a.profile = c->profiling;
+  a.profile = c->profiling >= 2;
   a.delta = c->cfg.delta;
   if ((rc = run_propagation(c, a, c->cfg.maxIterations)) != DLP_OK) return rc;
   if ((rc = dlp_batch_end(d, bt, s)) != DLP_OK) return rc;
@@ -719,7 +727,8 @@ extern "C" int dlp_context_set_profiling(dlp_context *c, int enable) {
   ctr.profLocalUpdates = ctr.profLocalEdges = ctr.profLocalChangedEdges = 0;
   ctr.profLocalAppended = 0;
   DLP_HIP(hipMemcpy(c->d.ctr, &ctr, sizeof(ctr), hipMemcpyHostToDevice));
-  c->profiling = enable ? 1 : 0;
+  c->profiling = enable;
+  c->timing.perChunk = enable == 1;
   c->timing.used = 0;
   c->timing.launches = c->timing.localLaunches = 0;
   c->timing.ms = c->timing.localMs = 0.0;

@@ -328,6 +328,7 @@ extern "C" int dlp_iterative_propagation(const int *rowPtr, const int *colInd,
   if (!w->ldsBytes && (rc = dlp_local_lds_bytes(&w->ldsBytes)) != DLP_OK) return rc;
   l.ldsWords = w->ldsBytes / 4;
   l.numNodes = numNodes;
+  l.shape = 0;
   l.coreListCap = l.ovfCap = dlp_local_list_cap(l.ldsWords);
   WS_BUF(int, coreGen, 19, numNodes);
   WS_BUF(int, coreIdx, 20, numNodes);

@@ -54,9 +54,49 @@ struct GlobalPos {
   }
 };
 
+constexpr int kPT = 512;       /* threads of the multi-block kernel          */
+constexpr int kStageQ = 4096;  /* per-block staged next-frontier appends     */
+constexpr int kStageT = 2048;  /* per-block staged touched-list appends      */
+
+/* Block-staged append: lanes with `pred` get a slot in the block's LDS stage
+ * (one LDS atomic per wave); entries past its capacity go straight to the
+ * global list.  The stage is flushed with one global atomic per block, so a
+ * launch issues about one same-address global atomic per block instead of
+ * one per wave and mark round. */
+template <int CAP>
+__device__ __forceinline__ void stage_append(int *stage, int *stageCount, int *gCount,
+                                             int *gList, bool pred, int v) {
+  const unsigned long long m = dlp_ballot(pred);
+  if (m == 0ull) return;
+  const int leader = __ffsll((long long)m) - 1;
+  int base = 0;
+  if (dlp_lane() == leader) base = atomicAdd(stageCount, __popcll(m));
+  base = __shfl(base, leader, 64);
+  const int idx = base + dlp_rank_below(m);
+  const bool spill = pred && idx >= CAP;
+  if (pred && !spill) stage[idx] = v;
+  const int slot = dlp_wave_append(gCount, spill);
+  if (slot >= 0) gList[slot] = v;
+}
+
+/* copies the block's stage [0, min(count, CAP)) to a range of the global
+ * list reserved with one atomic; all threads of the block, after a barrier */
+template <int CAP>
+__device__ __forceinline__ void stage_flush(const int *stage, int count, int *gCount, int *gList,
+                                            int *baseSlot) {
+  const int nb = min(count, CAP);
+  if (threadIdx.x == 0) *baseSlot = nb ? atomicAdd(gCount, nb) : 0;
+  __syncthreads();
+  const int b = *baseSlot;
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) gList[b + i] = stage[i];
+}
+
 template <int G>
-__global__ __launch_bounds__(256) void k_propagate(PropArgs a, int k) {
+__global__ __launch_bounds__(kPT) void k_propagate(PropArgs a, int k) {
   constexpr int GPW = 64 / G; /* frontier vertices per wave */
+  __shared__ int stageQ[kStageQ];
+  __shared__ int stageT[kStageT];
+  __shared__ int stageCount[2], stageBase[2];
   DevCounters *c = a.ctr;
   const int n = c->qcount[k % 3];
   if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -68,7 +108,9 @@ __global__ __launch_bounds__(256) void k_propagate(PropArgs a, int k) {
       c->updates += (unsigned long long)n;
     }
   }
-  if (n == 0) return;
+  if (n == 0) return;  // block-uniform
+  if (threadIdx.x == 0) stageCount[0] = stageCount[1] = 0;
+  __syncthreads();
 
   const int stampBase = c->stampBase;
   const int stampNext = stampBase + k + 1;
@@ -123,14 +165,15 @@ __global__ __launch_bounds__(256) void k_propagate(PropArgs a, int k) {
       const int old = pred ? atomicMax(&a.iterStamp[v], stampNext) : DLP_INT_MAX;
       const bool app = pred && old < stampNext;
       profA += app ? 1u : 0u;
-      const int slot = dlp_wave_append(outCount, app);
-      if (slot >= 0) out[slot] = v;
-      if (a.touched) {
-        const int ts = dlp_wave_append(&c->touched, app && old < stampBase);
-        if (ts >= 0) a.touched[ts] = v;
-      }
+      stage_append<kStageQ>(stageQ, &stageCount[0], outCount, out, app, v);
+      if (a.touched)
+        stage_append<kStageT>(stageT, &stageCount[1], &c->touched, a.touched,
+                              app && old < stampBase, v);
     }
   }
+  __syncthreads();
+  stage_flush<kStageQ>(stageQ, stageCount[0], outCount, out, &stageBase[0]);
+  if (a.touched) stage_flush<kStageT>(stageT, stageCount[1], &c->touched, a.touched, &stageBase[1]);
   if (a.profile) {  // one atomic per counter per wave
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) {
@@ -149,30 +192,32 @@ __global__ __launch_bounds__(256) void k_propagate(PropArgs a, int k) {
 /* ------------------------------------------------------------------------ */
 /* LDS-resident iterations (one work-group)                                 */
 /* ------------------------------------------------------------------------ */
-constexpr int kLT = 1024; /* threads of the local kernel             */
-constexpr int kLG = 4;    /* lanes per frontier vertex               */
-constexpr int kLGroups = kLT / kLG;
+/* Shapes of the local kernel: LT threads; rows of length <= SR are summed
+ * by one lane, longer rows by a wave.  Picked per context (tuning). */
+struct LocalShape {
+  int threads, shortRow;
+};
+constexpr LocalShape kLocalShapes[] = {{1024, 32}, {1024, 16}, {512, 32}, {256, 32}, {1024, 8}};
+constexpr int kNumLocalShapes = sizeof(kLocalShapes) / sizeof(kLocalShapes[0]);
 
-/* LDS edge record: target t (>= 0 core index, -1 contributes nothing,
- * <= -2 a vertex outside the core: -2 - (2*gid + eligible)), weight, and the
- * outside vertex's labels in both buffers (they cannot change while it is
- * outside the core) */
+/* LDS edge record {t, w, hA, hB}: target t (>= 0 core index, -1 contributes
+ * nothing, <= -2 a vertex outside the core: -2 - (2*gid + eligible)), weight
+ * w, and an outside vertex's labels in buffers 0 and 1 (they cannot change
+ * while it is outside the core).  w, hA and hB are float bit patterns. */
 struct LocalPos {
-  const int *etgt;
-  const float *ew;
-  const float *eh;  /* outside-vertex label in the buffer being read */
-  const float *Fo;  /* core labels in the buffer being read          */
+  const int4 *rec;
+  const float *Fo;  /* core labels in the buffer being read */
   int rs;
+  int kb;           /* buffer being read                    */
   __device__ void operator()(int p, float &w, float &s) const {
-    const int e = rs + p;
-    const int t = etgt[e];
-    if (t == -1) {
+    const int4 r = rec[rs + p];
+    if (r.x == -1) {
       w = 0.0f;
       s = 0.0f;
       return;
     }
-    w = ew[e];
-    const float F = t >= 0 ? Fo[t] : eh[e];
+    w = __int_as_float(r.y);
+    const float F = r.x >= 0 ? Fo[r.x] : __int_as_float(kb ? r.w : r.z);
     s = w * F;
   }
 };
@@ -188,10 +233,12 @@ __device__ __forceinline__ void wg_global_sync() {
   __threadfence();
 }
 
-/* exclusive scan of a[0..n) in LDS into out, returns the total; all threads */
+/* exclusive scan of a[0..n) in LDS into out, returns the total; all LT
+ * threads */
+template <int LT>
 __device__ int block_scan_lds(const int *a, int *out, int n, int *tmp /* >= 17 ints */) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int per = (n + kLT - 1) / kLT;
+  const int per = (n + LT - 1) / LT;
   const int lo = min(tid * per, n), hi = min(lo + per, n);
   int s = 0;
   for (int i = lo; i < hi; ++i) s += a[i];
@@ -205,7 +252,7 @@ __device__ int block_scan_lds(const int *a, int *out, int n, int *tmp /* >= 17 i
   __syncthreads();
   if (tid == 0) {
     int run = 0;
-    for (int w = 0; w < kLT / 64; ++w) {
+    for (int w = 0; w < LT / 64; ++w) {
       int t = tmp[w];
       tmp[w] = run;
       run += t;
@@ -225,24 +272,135 @@ __device__ int block_scan_lds(const int *a, int *out, int n, int *tmp /* >= 17 i
 }
 
 enum {
-  SH_GEN, SH_CC, SH_NCAND, SH_NCORE, SH_BIG, SH_REBUILD, SH_NOVF,
-  SH_QC0, SH_QC1, SH_QC2, SH_COUNT
+  SH_GEN, SH_CC, SH_NCAND, SH_NCORE, SH_BIG, SH_REBUILD, SH_NOVF, SH_HASLONG,
+  SH_QC0, SH_QC1, SH_QC2, SH_NLONG0, SH_NLONG1, SH_COUNT
 };
 
-__global__ __launch_bounds__(kLT) void k_propagate_local(PropArgs a, LocalArgs l, int k0,
-                                                         int maxIter) {
-  extern __shared__ int pool[];
+/* state of one local iteration shared by its short-row and long-row paths */
+struct LocalIter {
+  const int4 *rec;
+  int *stampL;
+  int *qout;
+  int *qcOut;
+  int *sh;
+  int stampNextL, stampNextG, stampBase;
+  int kb;  /* label buffer read by this iteration */
+};
+
+
+
+/* an eligible vertex outside the core was marked: global stamp, overflow
+ * list (it joins the core at the rebuild after this iteration), touched */
+__device__ __forceinline__ unsigned mark_outside(const PropArgs &a, const LocalArgs &l,
+                                                 const LocalIter &it, int gid) {
+  const int old = atomicMax(&a.iterStamp[gid], it.stampNextG);
+  if (old >= it.stampNextG) return 0u;
+  const int os = atomicAdd(&it.sh[SH_NOVF], 1);
+  if (os < l.ovfCap) l.ovf[os] = gid;
+  it.sh[SH_REBUILD] = 1;
+  if (a.touched && old < it.stampBase) a.touched[atomicAdd(&a.ctr->touched, 1)] = gid;
+  return 1u;
+}
+
+/* Frontier vertices whose rows have at most L entries, one per lane (all 64
+ * lanes call it; `act` lanes hold a vertex).  Loads are issued for every
+ * position 0..L-1 without branches (rows are padded by the LDS slack) and
+ * the mark atomics of all positions are issued before their results are
+ * used, so a lane waits for a few LDS round trips, not one per entry. */
+template <int L>
+__device__ __forceinline__ void local_short_rows(const PropArgs &a, const LocalArgs &l,
+                                                 const LocalIter &it, const float *Fo, float *Fn,
+                                                 bool act, int ci, int rs, int len,
+                                                 unsigned &profE, unsigned &profC,
+                                                 unsigned &profA) {
+  int tt[L];
+  float ww[L], hh[L], ff[L];
+#pragma unroll
+  for (int j = 0; j < L; ++j) {
+    const int4 r = it.rec[rs + j];
+    tt[j] = r.x;
+    ww[j] = __int_as_float(r.y);
+    hh[j] = __int_as_float(it.kb ? r.w : r.z);
+  }
+#pragma unroll
+  for (int j = 0; j < L; ++j) ff[j] = Fo[(j < len && tt[j] >= 0) ? tt[j] : ci];
+  const float oldLabel = Fo[ci];
+  float W, S;
+  dlp::lane_row_sums<L>(
+      len,
+      [&](int j, float &w, float &s) {
+        const bool contrib = tt[j] != -1;
+        const float F = tt[j] >= 0 ? ff[j] : hh[j];
+        w = contrib ? ww[j] : 0.0f;
+        s = contrib ? ww[j] * F : 0.0f;
+      },
+      W, S);
+  bool changed = false;
+  if (act) {
+    const float nl = dlp::new_label(W, S, oldLabel);
+    Fn[ci] = nl;
+    changed = fabsf(nl - oldLabel) > a.delta;
+    profE += (unsigned)len;
+    if (changed) profC += (unsigned)len + 1u;
+  }
+  /* marks: row positions 0..len-1 and the vertex itself as position len */
+  int olds[L + 1];
+  unsigned long long outside = 0ull;
+#pragma unroll
+  for (int p = 0; p <= L; ++p) {
+    const int t = (p < L && p < len) ? tt[p < L ? p : 0] : ci;
+    const bool valid = changed && p <= len;
+    int o = DLP_INT_MAX;
+    if (valid && t >= 0) o = atomicMax(&it.stampL[t], it.stampNextL);
+    olds[p] = o;
+    if (valid && t <= -2 && ((-2 - t) & 1)) outside |= 1ull << p;
+  }
+  unsigned long long hit = 0ull;
+#pragma unroll
+  for (int p = 0; p <= L; ++p)
+    if (olds[p] < it.stampNextL) hit |= 1ull << p;
+  while (outside) {  // rare: eligible vertices outside the core
+    const int p = __ffsll((long long)outside) - 1;
+    outside &= outside - 1;
+    profA += mark_outside(a, l, it, (-2 - it.rec[rs + p].x) >> 1);
+  }
+  /* one LDS atomic per wave reserves the wave's core appends */
+  const int cnt = __popcll(hit);
+  profA += (unsigned)cnt;
+  const int incl = dlp_wave_inclusive_sum(cnt);
+  const int total = __builtin_amdgcn_readlane(incl, 63);
+  if (total > 0) {  // wave-uniform
+    int wbase = 0;
+    if (dlp_lane() == 63) wbase = atomicAdd(it.qcOut, total);
+    wbase = __builtin_amdgcn_readlane(wbase, 63);
+    int pos = wbase + incl - cnt;
+#pragma unroll
+    for (int p = 0; p <= L; ++p) {
+      if ((hit >> p) & 1ull) it.qout[pos++] = (p < L && p < len) ? tt[p < L ? p : 0] : ci;
+    }
+  }
+}
+
+/* LDS words past the edge records: a row's branch-free gathers read up to
+ * 64 records past its start */
+constexpr int kLdsSlack = 4 * 64;
+
+template <int LT, int SR>
+__global__ __launch_bounds__(LT) void k_propagate_local(PropArgs a, LocalArgs l, int k0,
+                                                        int maxIter) {
+  extern __shared__ int4 pool4[];
+  int *pool = (int *)pool4;
   __shared__ int sh[SH_COUNT];
   __shared__ int tmp[17];
   DevCounters *c = a.ctr;
-  const int tid = threadIdx.x, lane = tid & 63;
-  const int gl = tid & (kLG - 1);
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int P = l.ldsWords;
   const int stampBase = c->stampBase;
   int k = k0;
   int reason = 0;
   bool useCore = true;
   unsigned profU = 0, profE = 0, profC = 0, profA = 0;
+  unsigned long long passUpdates = 0; /* tid 0: frontier sizes of this pass */
 
   while (true) {  // one pass = build the LDS working set, iterate, write back
     /* every thread has read the previous pass's sh[] values */
@@ -259,6 +417,9 @@ __global__ __launch_bounds__(kLT) void k_propagate_local(PropArgs a, LocalArgs l
       sh[SH_BIG] = 0;
       sh[SH_REBUILD] = 0;
       sh[SH_NOVF] = 0;
+      sh[SH_HASLONG] = 0;
+      sh[SH_NLONG0] = 0;
+      sh[SH_NLONG1] = 0;
     }
     __syncthreads();
     const int gen = sh[SH_GEN], cc = sh[SH_CC], nCand = sh[SH_NCAND];
@@ -272,7 +433,7 @@ __global__ __launch_bounds__(kLT) void k_propagate_local(PropArgs a, LocalArgs l
       break;
     }
     int *gidL = pool;  // [nCand]
-    for (int i = tid; i < nCand; i += kLT) {
+    for (int i = tid; i < nCand; i += LT) {
       const int g = i < cc ? l.coreList[i] : qk[i - cc];
       if (atomicMax(&l.coreGen[g], gen) < gen) {
         const int idx = atomicAdd(&sh[SH_NCORE], 1);
@@ -284,13 +445,17 @@ __global__ __launch_bounds__(kLT) void k_propagate_local(PropArgs a, LocalArgs l
     const int nCore = sh[SH_NCORE];
     int *rlL = gidL + nCand;
     int *rsL = rlL + nCore;
-    for (int i = tid; i < nCore; i += kLT) {
+    for (int i = tid; i < nCore; i += LT) {
       const int g = gidL[i];
-      rlL[i] = a.rowLen ? a.rowLen[g] : a.rowEnd[g] - a.rowBeg[g];
+      const int rl = a.rowLen ? a.rowLen[g] : a.rowEnd[g] - a.rowBeg[g];
+      rlL[i] = rl;
+      if (rl > SR) sh[SH_HASLONG] = 1;
     }
     __syncthreads();
-    const int E = block_scan_lds(rlL, rsL, nCore, tmp);
-    if ((long long)nCand + 7LL * nCore + 4LL * E > P) {  // uniform
+    const int E = block_scan_lds<LT>(rlL, rsL, nCore, tmp);  // ends with a barrier
+    const bool hasLong = sh[SH_HASLONG] != 0;
+    const int recOff = (nCand + 8 * nCore + 3) & ~3;  // 16-byte aligned records
+    if ((long long)recOff + 4LL * E + kLdsSlack > P) {  // uniform
       if (useCore && cc > 0) {
         useCore = false;
         continue;
@@ -303,12 +468,10 @@ __global__ __launch_bounds__(kLT) void k_propagate_local(PropArgs a, LocalArgs l
     int *stampL = (int *)(lab1L + nCore);
     int *q0L = stampL + nCore;
     int *q1L = q0L + nCore;
-    int *etgt = q1L + nCore;
-    float *ew = (float *)(etgt + E);
-    float *ehA = ew + E;
-    float *ehB = ehA + E;
+    int *longL = q1L + nCore;
+    int4 *rec = (int4 *)(pool + recOff);
     /* ---- stage core labels, stamps and rows ------------------------------ */
-    for (int i = tid; i < nCore; i += kLT) {
+    for (int i = tid; i < nCore; i += LT) {
       const int g = gidL[i];
       lab0L[i] = a.lab0[g];
       lab1L[i] = a.lab1[g];
@@ -316,26 +479,27 @@ __global__ __launch_bounds__(kLT) void k_propagate_local(PropArgs a, LocalArgs l
       const int beg = a.rowBeg[g], len = rlL[i], out = rsL[i];
       for (int p = 0; p < len; ++p) {
         const int v = a.colInd[beg + p];
-        const int e = out + p;
-        ew[e] = a.values[beg + p];
+        int4 r = make_int4(-1, __float_as_int(a.values[beg + p]), 0, 0);
         if (v < 0 || (a.status && a.status[v] == DLP_VERTEX_DELETED)) {
-          etgt[e] = -1;
+          r.x = -1;
         } else if (ld_agent(&l.coreGen[v]) == gen) {
-          etgt[e] = ld_agent(&l.coreIdx[v]);
+          r.x = ld_agent(&l.coreIdx[v]);
         } else {
           const int elig = a.iterStamp[v] != DLP_INT_MAX ? 1 : 0;
-          etgt[e] = -2 - (2 * v + elig);
-          ehA[e] = a.lab0[v];
-          ehB[e] = a.lab1[v];
+          r.x = -2 - (2 * v + elig);
+          r.z = __float_as_int(a.lab0[v]);
+          r.w = __float_as_int(a.lab1[v]);
         }
+        rec[out + p] = r;
       }
     }
+    for (int i = tid; i < kLdsSlack / 4; i += LT) rec[E + i] = make_int4(-1, 0, 0, 0);
     __syncthreads();
     /* ---- frontier of iteration k ----------------------------------------- */
     const int nq = nCand - cc;
     {
       int *qin = (k & 1) ? q1L : q0L;
-      for (int i = tid; i < nq; i += kLT) {
+      for (int i = tid; i < nq; i += LT) {
         const int idx = ld_agent(&l.coreIdx[qk[i]]);
         qin[i] = idx;
         stampL[idx] = k;
@@ -360,70 +524,83 @@ __global__ __launch_bounds__(kLT) void k_propagate_local(PropArgs a, LocalArgs l
         reason = DLP_LOCAL_CAPPED;
         break;
       }
+      const int kb = k & 1;
       if (tid == 0) {
-        sh[SH_QC0 + (k + 2) % 3] = 0;
+        sh[SH_QC0 + (k + 2) % 3] = 0;    // appended to by iteration k+1
+        sh[SH_NLONG0 + (kb ^ 1)] = 0;    // long-row list of iteration k+1
         if (k == 0) c->frontier0 = n;
         if (a.iterSizes && k < a.iterSizesCap) a.iterSizes[k] = n;
-        c->iterUsed = k + 1;
-        c->updates += (unsigned long long)n;
+        passUpdates += (unsigned long long)n;
         profU += (unsigned)n;
       }
-      const int kb = k & 1;
       const int *qin = kb ? q1L : q0L;
-      int *qout = kb ? q0L : q1L;
       const float *Fo = kb ? lab1L : lab0L;
       float *Fn = kb ? lab0L : lab1L;
-      const float *eh = kb ? ehB : ehA;
-      int *qcOut = &sh[SH_QC0 + (k + 1) % 3];
-      const int stampNextL = k + 1;
-      const int stampNextG = stampBase + k + 1;
-      for (int base = 0; base < n; base += kLGroups) {  // block-uniform
-        const int item = base + (tid >> 2);
+      int *nLongCtr = &sh[SH_NLONG0 + kb];
+      const LocalIter it{rec, stampL, kb ? q0L : q1L, &sh[SH_QC0 + (k + 1) % 3], sh,
+                         k + 1, stampBase + k + 1, stampBase, kb};
+
+      /* short rows: one lane per frontier vertex, the wave's longest row
+       * picks the unrolled width */
+      for (int base = 0; base < n; base += LT) {  // block-uniform
+        if (base + (tid & ~63) >= n) continue;     // wave-uniform: no vertex for this wave
+        const int item = base + tid;
         const bool act = item < n;
         const int ci = act ? qin[item] : 0;
         const int rs = act ? rsL[ci] : 0;
         const int len = act ? rlL[ci] : 0;
-        float W, S;
-        dlp::group_row_sums<kLG>(gl, len, LocalPos{etgt, ew, eh, Fo, rs}, W, S);
-        int changed = 0;
-        if (act && gl == 0) {
-          const float oldLabel = Fo[ci];
-          const float nl = dlp::new_label(W, S, oldLabel);
-          Fn[ci] = nl;
-          changed = fabsf(nl - oldLabel) > a.delta;
-          profE += (unsigned)len;
-          if (changed) profC += (unsigned)len + 1u;
+        const bool shortRow = act && len <= SR;
+        if (hasLong) {
+          const int slot = dlp_wave_append(nLongCtr, act && !shortRow);
+          if (slot >= 0) longL[slot] = ci;
         }
-        changed = __shfl(changed, lane & ~(kLG - 1), 64);
-        for (int p0 = 0; __any(changed && p0 <= len); p0 += kLG) {  // wave-uniform
-          const int p = p0 + gl;
-          int t = -1;
-          if (changed && p < len) t = etgt[rs + p];
-          else if (changed && p == len) t = ci;  // the vertex itself
-          bool appL = false, appG = false, firstG = false;
-          int gid = 0;
-          if (t >= 0) {
-            appL = atomicMax(&stampL[t], stampNextL) < stampNextL;
-          } else if (t <= -2) {
-            const int x = -2 - t;
-            if (x & 1) {  // eligible vertex outside the core: rebuild after this iteration
-              gid = x >> 1;
-              const int old = atomicMax(&a.iterStamp[gid], stampNextG);
-              appG = old < stampNextG;
-              firstG = appG && old < stampBase;
+        const int lenS = shortRow ? len : 0;
+        if (!__any(lenS > 4)) {
+          local_short_rows<4>(a, l, it, Fo, Fn, shortRow, ci, rs, lenS, profE, profC, profA);
+        } else if (SR <= 8 || !__any(lenS > 8)) {
+          local_short_rows<(SR < 8 ? SR : 8)>(a, l, it, Fo, Fn, shortRow, ci, rs, lenS, profE,
+                                              profC, profA);
+        } else if (SR <= 16 || !__any(lenS > 16)) {
+          local_short_rows<(SR < 16 ? SR : 16)>(a, l, it, Fo, Fn, shortRow, ci, rs, lenS, profE,
+                                                profC, profA);
+        } else {
+          local_short_rows<SR>(a, l, it, Fo, Fn, shortRow, ci, rs, lenS, profE, profC, profA);
+        }
+      }
+      /* long rows: one wave per frontier vertex */
+      if (hasLong) {  // uniform
+        __syncthreads();
+        const int nLong = *nLongCtr;
+        for (int i = wave; i < nLong; i += LT / 64) {  // wave-uniform
+          const int ci = longL[i];
+          const int rs = rsL[ci], len = rlL[ci];
+          float W, S;
+          dlp::group_row_sums<64>(lane, len, LocalPos{rec, Fo, rs, kb}, W, S);
+          int changed = 0;
+          if (lane == 0) {
+            const float oldLabel = Fo[ci];
+            const float nl = dlp::new_label(W, S, oldLabel);
+            Fn[ci] = nl;
+            changed = fabsf(nl - oldLabel) > a.delta;
+            profE += (unsigned)len;
+            if (changed) profC += (unsigned)len + 1u;
+          }
+          changed = __shfl(changed, 0, 64);
+          if (!changed) continue;  // wave-uniform
+          for (int p0 = 0; p0 <= len; p0 += 64) {  // wave-uniform
+            const int p = p0 + lane;
+            int t = -1;
+            if (p < len) t = rec[rs + p].x;
+            else if (p == len) t = ci;  // the vertex itself
+            bool appL = false;
+            if (t >= 0) {
+              appL = atomicMax(&stampL[t], it.stampNextL) < it.stampNextL;
+            } else if (t <= -2 && ((-2 - t) & 1)) {
+              profA += mark_outside(a, l, it, (-2 - t) >> 1);
             }
-          }
-          profA += (appL || appG) ? 1u : 0u;
-          const int slot = dlp_wave_append(qcOut, appL);
-          if (slot >= 0) qout[slot] = t;
-          const int os = dlp_wave_append(&sh[SH_NOVF], appG);
-          if (os >= 0) {
-            if (os < l.ovfCap) l.ovf[os] = gid;
-            sh[SH_REBUILD] = 1;
-          }
-          if (a.touched) {
-            const int ts = dlp_wave_append(&c->touched, firstG);
-            if (ts >= 0) a.touched[ts] = gid;
+            profA += appL ? 1u : 0u;
+            const int slot = dlp_wave_append(it.qcOut, appL);
+            if (slot >= 0) it.qout[slot] = t;
           }
         }
       }
@@ -433,7 +610,7 @@ __global__ __launch_bounds__(kLT) void k_propagate_local(PropArgs a, LocalArgs l
     }
 
     /* ---- write back: labels, touched, next frontier, core list ---------- */
-    for (int base = 0; base < nCore; base += kLT) {  // block-uniform
+    for (int base = 0; base < nCore; base += LT) {  // block-uniform
       const int i = base + tid;
       bool first = false;
       int g = 0;
@@ -458,12 +635,17 @@ __global__ __launch_bounds__(kLT) void k_propagate_local(PropArgs a, LocalArgs l
       const int no = min(sh[SH_NOVF], l.ovfCap);
       const int *qloc = (k & 1) ? q1L : q0L;
       int *gq = (k & 1) ? a.q1 : a.q0;
-      for (int i = tid; i < nl; i += kLT) gq[i] = gidL[qloc[i]];
-      for (int i = tid; i < no; i += kLT) gq[nl + i] = l.ovf[i];
+      for (int i = tid; i < nl; i += LT) gq[i] = gidL[qloc[i]];
+      for (int i = tid; i < no; i += LT) gq[nl + i] = l.ovf[i];
       if (tid == 0) {
         c->qcount[k % 3] = nl + no;
         c->qcount[(k + 1) % 3] = 0;
         c->coreCount = min(nCore, l.coreListCap);
+        if (k > kBuild) {  // iterations kBuild .. k-1 ran
+          c->iterUsed = k;
+          c->updates += passUpdates;
+        }
+        passUpdates = 0;
         /* cannot happen with ovfCap >= ldsWords / 4 (outside targets <= E) */
         if (sh[SH_NOVF] > l.ovfCap) {
           atomicOr(&c->errorFlag, 8);
@@ -502,7 +684,7 @@ __global__ __launch_bounds__(kLT) void k_propagate_local(PropArgs a, LocalArgs l
 }  // namespace
 
 int dlp_launch_propagate(const PropArgs &a, int k, int groupLanes, int blocks, hipStream_t s) {
-  dim3 grid(blocks > 0 ? blocks : 1), block(256);
+  dim3 grid(blocks > 0 ? blocks : 1), block(kPT);
   switch (groupLanes) {
     case 4: hipLaunchKernelGGL(k_propagate<4>, grid, block, 0, s, a, k); break;
     case 8: hipLaunchKernelGGL(k_propagate<8>, grid, block, 0, s, a, k); break;
@@ -517,6 +699,10 @@ int dlp_launch_propagate(const PropArgs &a, int k, int groupLanes, int blocks, h
   return DLP_OK;
 }
 
+#define DLP_LOCAL_SHAPES(X) X(0, 1024, 32) X(1, 1024, 16) X(2, 512, 32) X(3, 256, 32) X(4, 1024, 8)
+
+int dlp_local_shape_count() { return kNumLocalShapes; }
+
 int dlp_local_lds_bytes(int *bytes) {
   int dev = 0, v = 0;
   DLP_HIP(hipGetDevice(&dev));
@@ -527,16 +713,30 @@ int dlp_local_lds_bytes(int *bytes) {
     return DLP_ERR_INVALID;
   }
   v &= ~255;
-  DLP_HIP(hipFuncSetAttribute((const void *)k_propagate_local,
+#define DLP_SET_LDS(I, T, R)                                                       \
+  DLP_HIP(hipFuncSetAttribute((const void *)k_propagate_local<T, R>,              \
                               hipFuncAttributeMaxDynamicSharedMemorySize, v));
+  DLP_LOCAL_SHAPES(DLP_SET_LDS)
+#undef DLP_SET_LDS
   *bytes = v;
   return DLP_OK;
 }
 
 int dlp_launch_propagate_local(const PropArgs &a, const LocalArgs &l, int k, int maxIter,
                                hipStream_t s) {
-  hipLaunchKernelGGL(k_propagate_local, dim3(1), dim3(kLT), (size_t)l.ldsWords * 4, s, a, l, k,
-                     maxIter);
+  const size_t lds = (size_t)l.ldsWords * 4;
+  switch (l.shape) {
+#define DLP_LAUNCH_LOCAL(I, T, R)                                                       \
+  case I:                                                                              \
+    static_assert(kLocalShapes[I].threads == T && kLocalShapes[I].shortRow == R, "");  \
+    hipLaunchKernelGGL((k_propagate_local<T, R>), dim3(1), dim3(T), lds, s, a, l, k, maxIter); \
+    break;
+    DLP_LOCAL_SHAPES(DLP_LAUNCH_LOCAL)
+#undef DLP_LAUNCH_LOCAL
+    default:
+      dlp_set_errorf_("local propagation shape %d not in [0, %d)", l.shape, kNumLocalShapes);
+      return DLP_ERR_INVALID;
+  }
   DLP_LAUNCHED("k_propagate_local");
   return DLP_OK;
 }

@@ -255,10 +255,11 @@ class Context:
         check(self._lib.dlp_context_device_bytes(self._h, C.byref(b)))
         return b.value
 
-    def set_tuning(self, groupLanes=0, blocks=0, chunk=0, localThreshold=0, localLdsBytes=0):
+    def set_tuning(self, groupLanes=0, blocks=0, chunk=0, localThreshold=0, localLdsBytes=0,
+                   localShape=0):
         """0 keeps a setting; localThreshold < 0 disables the LDS-resident
-        propagation kernel."""
-        t = _lib.Tuning(groupLanes, blocks, chunk, localThreshold, localLdsBytes)
+        propagation kernel; localShape is 1-based."""
+        t = _lib.Tuning(groupLanes, blocks, chunk, localThreshold, localLdsBytes, localShape)
         check(self._lib.dlp_context_set_tuning(self._h, C.byref(t)))
 
     def tuning(self) -> dict:
@@ -275,8 +276,11 @@ class Context:
         check(self._lib.dlp_context_set_trace(self._h, None))
         self._trace = None
 
-    def set_profiling(self, enable: bool = True):
-        check(self._lib.dlp_context_set_profiling(self._h, 1 if enable else 0))
+    def set_profiling(self, level=True):
+        """0/False off, 1 event timing per local launch and per multi-block
+        chunk, 2/True per-launch events plus work counters."""
+        lv = 2 if level is True else (0 if level is False else int(level))
+        check(self._lib.dlp_context_set_profiling(self._h, lv))
 
     def prop_profile(self) -> dict:
         p = _lib.PropProfile()

@@ -116,21 +116,11 @@ def test_work_shapes_give_identical_results(gpu_available, c1_inputs, lanes, blo
         assert_labels_equal(ctx.labels(), orc.labels(), f"G={lanes} blocks={blocks}")
 
 
-@pytest.mark.parametrize("threshold,ldsBytes", [
-    (-1, 0),          # multi-block kernel only
-    (1 << 30, 0),     # LDS-resident kernel whenever the working set fits
-    (64, 0),          # small frontiers only: many engine switches
-    (1 << 30, 4096),  # 1024 LDS words: core rebuilds, retries, fallbacks
-    (200, 16384),
-])
-def test_engine_mix_gives_identical_results(gpu_available, c1_inputs, threshold, ldsBytes):
-    """The multi-block and the LDS-resident propagation kernels, in any mix,
-    leave the oracle's labels, iteration counts and frontier sizes."""
-    inp = c1_inputs
-    cfg = L.make_config(inp, maxIterations=3000)
-    orc = Oracle(inp, maxIterations=3000)
+def run_engine_mix(inp, maxIter, **tuning):
+    cfg = L.make_config(inp, maxIterations=maxIter)
+    orc = Oracle(inp, maxIterations=maxIter)
     with L.Context(inp, cfg) as ctx:
-        ctx.set_tuning(localThreshold=threshold, localLdsBytes=ldsBytes)
+        ctx.set_tuning(**tuning)
         ctx.set_profiling(True)
         tr = ctx.enable_trace(maxFrontierSizes=1 << 12)
         ctx.reset()
@@ -144,11 +134,38 @@ def test_engine_mix_gives_identical_results(gpu_available, c1_inputs, threshold,
             assert np.array_equal(snap["frontierSizes"][:n], fs[:n]), f"batch {b}: frontiers"
             assert_labels_equal(ctx.labels(), orc.labels(), f"batch {b}: labels")
             assert ctx.stats(b)["updates"] == orc.stats()["updates"]
-        prof = ctx.prop_profile()
+        return ctx.prop_profile()
+
+
+@pytest.mark.parametrize("threshold,ldsBytes,shape", [
+    (-1, 0, 1),          # multi-block kernel only
+    (1 << 30, 0, 1),     # LDS-resident kernel whenever the working set fits
+    (64, 0, 1),          # small frontiers only: many engine switches
+    (1 << 30, 4096, 1),  # 1024 LDS words: core rebuilds, retries, fallbacks
+    (200, 16384, 1),
+    (1 << 30, 0, 2), (1 << 30, 0, 3), (1 << 30, 0, 4),
+    (1 << 30, 0, 5),     # one lane per row up to 8 entries: wave path for the rest
+])
+def test_engine_mix_gives_identical_results(gpu_available, c1_inputs, threshold, ldsBytes,
+                                            shape):
+    """The multi-block and the LDS-resident propagation kernels, in any mix
+    and work-group shape, leave the oracle's labels, iteration counts and
+    frontier sizes."""
+    prof = run_engine_mix(c1_inputs, 3000, localThreshold=threshold, localLdsBytes=ldsBytes,
+                          localShape=shape)
     if threshold < 0:
         assert prof["localLaunches"] == 0
     else:
         assert prof["localLaunches"] > 0
+
+
+@pytest.mark.parametrize("shape", [1, 2, 5])
+def test_engine_mix_long_rows(gpu_available, shape):
+    """Average degree 40: rows longer than 32 take the wave-per-row paths of
+    both kernels (and the general row sum beyond 64 entries)."""
+    inp = L.generate(3000, 40, 6, 7)
+    prof = run_engine_mix(inp, 2000, localThreshold=1 << 30, localShape=shape)
+    assert prof["localLaunches"] > 0
 
 
 def test_replay_is_deterministic(gpu_available, c1_inputs):
