@@ -218,10 +218,9 @@ typedef struct dlp_tuning {
                            < 0 disables it                               */
   int localLdsBytes;    /* LDS the local kernel may use, at most the
                            device's per-work-group limit (the default)   */
-  int localShape;       /* work-group shape of the local kernel, 1-based:
-                           1 = 1024 threads, one lane per row of <= 32
-                           entries; 2 = 1024/16; 3 = 512/32; 4 = 256/32;
-                           5 = 1024/8                                    */
+  int localShape;       /* work-group shape of the local kernel, 1-based
+                           (threads / records per padded row): 1 = 1024/8,
+                           2 = 1024/16, 3 = 512/8, 4 = 256/8, 5 = 1024/4 */
 } dlp_tuning;
 
 /* Uploads the full graph, true labels and every batch, allocates all device

@@ -40,6 +40,10 @@ struct LocalArgs {
   int *coreGen;          /* N: generation in which the vertex is in the core;
                             zero-filled at allocation, with ctr->localGen = 0  */
   int *coreIdx;          /* N: its core index in that generation               */
+  int *haloGen;          /* N: generation in which the vertex is a neighbour of
+                            the core outside it (zero-filled like coreGen)     */
+  int *haloIdx;          /* N: its halo index in that generation (bit 30:
+                            ineligible)                                        */
   int *coreList;         /* capacity coreListCap: core of the last build       */
   int coreListCap;
   int *ovf;              /* capacity ovfCap: marked vertices outside the core  */
@@ -49,6 +53,13 @@ struct LocalArgs {
 };
 /* allocation sizes of the LocalArgs buffers for ldsWords words of LDS */
 inline int dlp_local_list_cap(int ldsWords) { return ldsWords / 4 + 64; }
+/* default frontier threshold of the local kernel: the frontier whose
+ * working set (about 26 words per core vertex with rows padded to 8
+ * records, plus labels of its neighbours outside the core) fills the LDS */
+inline int dlp_local_default_threshold(int ldsWords, long long meanDeg) {
+  const long long t = ldsWords / (26 + 2 * meanDeg);
+  return (int)(t < 1 ? 1 : t);
+}
 
 /* Runs propagation iterations from k in one work-group with the frontier's
  * rows and neighbour labels held in LDS, until the frontier empties, k
@@ -60,8 +71,8 @@ int dlp_launch_propagate_local(const PropArgs &a, const LocalArgs &l, int k, int
                                hipStream_t s);
 /* dynamic LDS bytes the local kernel can use on the current device */
 int dlp_local_lds_bytes(int *bytes);
-/* number of work-group shapes of the local kernel (0 = threads 1024, one
- * lane per row up to 32 entries) */
+/* number of work-group shapes of the local kernel (0 = 1024 threads, rows
+ * padded to 8 records) */
 int dlp_local_shape_count();
 
 /* HIP-event timing of propagation launches (optional) */

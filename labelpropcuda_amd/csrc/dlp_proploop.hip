@@ -88,6 +88,7 @@ int dlp_run_propagation(const PropArgs &a, const LocalArgs *l, int maxIter,
     if (l && n <= cfg.localThreshold && k >= localAfter) {
       if (pin->localGen > (1 << 30)) {  // restart the core generations
         DLP_HIP(hipMemsetAsync(l->coreGen, 0, (size_t)l->numNodes * sizeof(int), s));
+        DLP_HIP(hipMemsetAsync(l->haloGen, 0, (size_t)l->numNodes * sizeof(int), s));
         DLP_HIP(hipMemsetAsync(&a.ctr->localGen, 0, sizeof(int), s));
       }
       if ((rc = begin_timed(timing, s, true, 1)) != DLP_OK) return rc;

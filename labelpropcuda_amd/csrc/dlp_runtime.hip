@@ -500,13 +500,16 @@ extern "C" int dlp_context_create(dlp_context **out, const dlp_config *cfg,
     l.numNodes = N;
     l.coreListCap = l.ovfCap = dlp_local_list_cap(l.ldsWords);
     TRY(c->alloc(&l.coreGen, N));
+    TRY(c->alloc(&l.haloGen, N));
+    TRY(c->alloc(&l.haloIdx, N));
     TRY(c->alloc(&l.coreIdx, N));
     TRY(c->alloc(&l.coreList, l.coreListCap));
     TRY(c->alloc(&l.ovf, l.ovfCap));
     DLP_HIP(hipMemsetAsync(l.coreGen, 0, (size_t)N * sizeof(int), c->stream));
+    DLP_HIP(hipMemsetAsync(l.haloGen, 0, (size_t)N * sizeof(int), c->stream));
     DLP_HIP(hipMemsetAsync(d.ctr, 0, sizeof(DevCounters), c->stream));
     const long long meanDeg = (g->numEdges + N - 1) / N;
-    c->tuning.localThreshold = (int)std::max<long long>(1, l.ldsWords / (8 + 4 * meanDeg));
+    c->tuning.localThreshold = dlp_local_default_threshold(l.ldsWords, meanDeg);
     c->tuning.localLdsBytes = c->localMaxBytes = ldsBytes;
   }
 #undef TRY

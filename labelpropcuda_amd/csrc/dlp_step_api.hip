@@ -330,8 +330,9 @@ extern "C" int dlp_iterative_propagation(const int *rowPtr, const int *colInd,
   l.numNodes = numNodes;
   l.shape = 0;
   l.coreListCap = l.ovfCap = dlp_local_list_cap(l.ldsWords);
-  WS_BUF(int, coreGen, 19, numNodes);
+  WS_BUF(int, coreGen, 19, 2LL * numNodes);  // core generations, then halo generations
   WS_BUF(int, coreIdx, 20, numNodes);
+  WS_BUF(int, haloIdx, 23, numNodes);
   WS_BUF(int, coreList, 21, l.coreListCap);
   WS_BUF(int, ovf, 22, l.ovfCap);
   if (coreGen != w->coreGenBuf || w->sizes[19] != w->coreGenBytes) {
@@ -344,7 +345,9 @@ extern "C" int dlp_iterative_propagation(const int *rowPtr, const int *colInd,
     w->localGen = 0;
   }
   l.coreGen = coreGen;
+  l.haloGen = coreGen + numNodes;
   l.coreIdx = coreIdx;
+  l.haloIdx = haloIdx;
   l.coreList = coreList;
   l.ovf = ovf;
   PropLoopConfig cfg;
@@ -356,7 +359,7 @@ extern "C" int dlp_iterative_propagation(const int *rowPtr, const int *colInd,
     DLP_HIP(hipMemcpyAsync(&nnz, rowPtr + numNodes, sizeof(int), hipMemcpyDeviceToHost, s));
     DLP_HIP(hipStreamSynchronize(s));
     const long long meanDeg = ((long long)std::max(nnz, 0) + numNodes - 1) / numNodes;
-    cfg.localThreshold = (int)std::max<long long>(1, l.ldsWords / (8 + 4 * meanDeg));
+    cfg.localThreshold = dlp_local_default_threshold(l.ldsWords, meanDeg);
   }
   rc = dlp_run_propagation(a, numNodes < (1 << 30) ? &l : nullptr, maxIterations, cfg, w->pin, s,
                            nullptr);

@@ -192,33 +192,34 @@ __global__ __launch_bounds__(kPT) void k_propagate(PropArgs a, int k) {
 /* ------------------------------------------------------------------------ */
 /* LDS-resident iterations (one work-group)                                 */
 /* ------------------------------------------------------------------------ */
-/* Shapes of the local kernel: LT threads; rows of length <= SR are summed
- * by one lane, longer rows by a wave.  Picked per context (tuning). */
+/* Work-group shapes of the local kernel: LT threads; rows of up to SR
+ * entries are stored padded to SR records, so one lane reads them without
+ * per-entry masks.  Picked per context (tuning). */
 struct LocalShape {
-  int threads, shortRow;
+  int threads, stride;
 };
-constexpr LocalShape kLocalShapes[] = {{1024, 32}, {1024, 16}, {512, 32}, {256, 32}, {1024, 8}};
+constexpr LocalShape kLocalShapes[] = {{1024, 8}, {1024, 16}, {512, 8}, {256, 8}, {1024, 4}};
 constexpr int kNumLocalShapes = sizeof(kLocalShapes) / sizeof(kLocalShapes[0]);
 
-/* LDS edge record {t, w, hA, hB}: target t (>= 0 core index, -1 contributes
- * nothing, <= -2 a vertex outside the core: -2 - (2*gid + eligible)), weight
- * w, and an outside vertex's labels in buffers 0 and 1 (they cannot change
- * while it is outside the core).  w, hA and hB are float bit patterns. */
+/* Rows up to this length are summed by one lane (canonical tree over 16
+ * slots); longer rows by a wave. */
+constexpr int kLaneRowMax = 16;
+
+/* The working set of a pass.  Label slots: [0, nCore) core vertices,
+ * [nCore, nCore + nHE) eligible vertices outside the core (halo), then the
+ * ineligible halo, then one null slot whose labels are 0.  An LDS row entry
+ * is a record {slot, weight bits}; holes and entries pointing at deleted
+ * vertices are {null, 0}, so every entry is summed as w * F[slot] with no
+ * branch (0 * 0 adds +0.0, which changes no partial sum).  Halo labels do
+ * not change while a vertex is outside the core. */
 struct LocalPos {
-  const int4 *rec;
-  const float *Fo;  /* core labels in the buffer being read */
+  const int2 *rec;
+  const float *Fo;
   int rs;
-  int kb;           /* buffer being read                    */
   __device__ void operator()(int p, float &w, float &s) const {
-    const int4 r = rec[rs + p];
-    if (r.x == -1) {
-      w = 0.0f;
-      s = 0.0f;
-      return;
-    }
+    const int2 r = rec[rs + p];
     w = __int_as_float(r.y);
-    const float F = r.x >= 0 ? Fo[r.x] : __int_as_float(kb ? r.w : r.z);
-    s = w * F;
+    s = w * Fo[r.x];
   }
 };
 
@@ -233,8 +234,8 @@ __device__ __forceinline__ void wg_global_sync() {
   __threadfence();
 }
 
-/* exclusive scan of a[0..n) in LDS into out, returns the total; all LT
- * threads */
+/* exclusive scan of a[0..n) in LDS into out (may alias a), returns the
+ * total; all LT threads */
 template <int LT>
 __device__ int block_scan_lds(const int *a, int *out, int n, int *tmp /* >= 17 ints */) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -273,21 +274,21 @@ __device__ int block_scan_lds(const int *a, int *out, int n, int *tmp /* >= 17 i
 
 enum {
   SH_GEN, SH_CC, SH_NCAND, SH_NCORE, SH_BIG, SH_REBUILD, SH_NOVF, SH_HASLONG,
-  SH_QC0, SH_QC1, SH_QC2, SH_NLONG0, SH_NLONG1, SH_COUNT
+  SH_NHE, SH_NHI, SH_QC0, SH_QC1, SH_QC2, SH_NLONG0, SH_NLONG1, SH_COUNT
 };
 
-/* state of one local iteration shared by its short-row and long-row paths */
+/* state of one local iteration shared by its row paths */
 struct LocalIter {
-  const int4 *rec;
   int *stampL;
   int *qout;
   int *qcOut;
   int *sh;
+  const int *haloE;  /* gid of each eligible halo slot (slot - nCore) */
+  int *dummyStamp;   /* 64 words, INT_MAX: the mark target of idle lanes */
+  int *dummyOut;     /* 64 words: the queue write target of idle lanes   */
+  int nCore, nHE;
   int stampNextL, stampNextG, stampBase;
-  int kb;  /* label buffer read by this iteration */
 };
-
-
 
 /* an eligible vertex outside the core was marked: global stamp, overflow
  * list (it joins the core at the rebuild after this iteration), touched */
@@ -302,96 +303,113 @@ __device__ __forceinline__ unsigned mark_outside(const PropArgs &a, const LocalA
   return 1u;
 }
 
-/* Frontier vertices whose rows have at most L entries, one per lane (all 64
- * lanes call it; `act` lanes hold a vertex).  Loads are issued for every
- * position 0..L-1 without branches (rows are padded by the LDS slack) and
- * the mark atomics of all positions are issued before their results are
- * used, so a lane waits for a few LDS round trips, not one per entry. */
-template <int L>
-__device__ __forceinline__ void local_short_rows(const PropArgs &a, const LocalArgs &l,
-                                                 const LocalIter &it, const float *Fo, float *Fn,
-                                                 bool act, int ci, int rs, int len,
-                                                 unsigned &profE, unsigned &profC,
-                                                 unsigned &profA) {
+/* One frontier vertex per lane (all 64 lanes call it; `act` lanes hold a
+ * vertex whose row has at most L entries).  The row's records start at
+ * `row`; without MASKED every position 0..L-1 holds a record (padding is
+ * {null, 0}), with MASKED positions >= len are skipped.  Marks and queue
+ * writes are branch-free: a lane with nothing to mark or write at a
+ * position targets its own dummy LDS word (a stamp that stays INT_MAX, a
+ * scratch slot), so a lane waits for a few LDS round trips, not one per
+ * entry, and no exec-mask juggling is needed. */
+template <int L, bool MASKED>
+__device__ __forceinline__ void lane_rows(const PropArgs &a, const LocalArgs &l,
+                                          const LocalIter &it, const int2 *row, int nullSlot,
+                                          const float *Fo, float *Fn, bool act, int ci, int len,
+                                          bool hasOut, unsigned &profE, unsigned &profC,
+                                          unsigned &profA) {
+  static_assert(L + 1 <= 32, "hit mask is 32 bits");
   int tt[L];
-  float ww[L], hh[L], ff[L];
+  float ww[L], ff[L];
 #pragma unroll
   for (int j = 0; j < L; ++j) {
-    const int4 r = it.rec[rs + j];
+    int2 r = make_int2(nullSlot, 0);
+    if (!MASKED || j < len) r = row[j];
     tt[j] = r.x;
     ww[j] = __int_as_float(r.y);
-    hh[j] = __int_as_float(it.kb ? r.w : r.z);
   }
 #pragma unroll
-  for (int j = 0; j < L; ++j) ff[j] = Fo[(j < len && tt[j] >= 0) ? tt[j] : ci];
+  for (int j = 0; j < L; ++j) ff[j] = Fo[tt[j]];
   const float oldLabel = Fo[ci];
   float W, S;
   dlp::lane_row_sums<L>(
-      len,
+      L,
       [&](int j, float &w, float &s) {
-        const bool contrib = tt[j] != -1;
-        const float F = tt[j] >= 0 ? ff[j] : hh[j];
-        w = contrib ? ww[j] : 0.0f;
-        s = contrib ? ww[j] * F : 0.0f;
+        w = ww[j];
+        s = ww[j] * ff[j];
       },
       W, S);
-  bool changed = false;
+  const float nl = dlp::new_label(W, S, oldLabel);
+  const bool changed = act && fabsf(nl - oldLabel) > a.delta;
   if (act) {
-    const float nl = dlp::new_label(W, S, oldLabel);
     Fn[ci] = nl;
-    changed = fabsf(nl - oldLabel) > a.delta;
     profE += (unsigned)len;
     if (changed) profC += (unsigned)len + 1u;
   }
-  /* marks: row positions 0..len-1 and the vertex itself as position len */
+  /* marks: the row's core targets (bits 0..L-1) and the vertex itself (bit
+   * L) */
+  const int lane = dlp_lane();
+  int *dumStamp = it.dummyStamp + lane;
+  const int nCore = it.nCore, next = it.stampNextL;
   int olds[L + 1];
-  unsigned long long outside = 0ull;
 #pragma unroll
-  for (int p = 0; p <= L; ++p) {
-    const int t = (p < L && p < len) ? tt[p < L ? p : 0] : ci;
-    const bool valid = changed && p <= len;
-    int o = DLP_INT_MAX;
-    if (valid && t >= 0) o = atomicMax(&it.stampL[t], it.stampNextL);
-    olds[p] = o;
-    if (valid && t <= -2 && ((-2 - t) & 1)) outside |= 1ull << p;
-  }
-  unsigned long long hit = 0ull;
+  for (int j = 0; j < L; ++j)
+    olds[j] = atomicMax((changed && tt[j] < nCore) ? &it.stampL[tt[j]] : dumStamp, next);
+  olds[L] = atomicMax(changed ? &it.stampL[ci] : dumStamp, next);
+  unsigned hit = 0u;
 #pragma unroll
-  for (int p = 0; p <= L; ++p)
-    if (olds[p] < it.stampNextL) hit |= 1ull << p;
-  while (outside) {  // rare: eligible vertices outside the core
-    const int p = __ffsll((long long)outside) - 1;
-    outside &= outside - 1;
-    profA += mark_outside(a, l, it, (-2 - it.rec[rs + p].x) >> 1);
+  for (int j = 0; j <= L; ++j) hit |= (olds[j] < next ? 1u : 0u) << j;
+  /* eligible vertices outside the core (rows flagged at the build) */
+  if (__any(changed && hasOut)) {  // wave-uniform
+    if (changed && hasOut) {
+#pragma unroll
+      for (int j = 0; j < L; ++j) {
+        const int h = tt[j] - nCore;
+        if ((unsigned)h < (unsigned)it.nHE) profA += mark_outside(a, l, it, it.haloE[h]);
+      }
+    }
   }
   /* one LDS atomic per wave reserves the wave's core appends */
-  const int cnt = __popcll(hit);
+  const int cnt = __popc(hit);
   profA += (unsigned)cnt;
   const int incl = dlp_wave_inclusive_sum(cnt);
   const int total = __builtin_amdgcn_readlane(incl, 63);
   if (total > 0) {  // wave-uniform
     int wbase = 0;
-    if (dlp_lane() == 63) wbase = atomicAdd(it.qcOut, total);
+    if (lane == 63) wbase = atomicAdd(it.qcOut, total);
     wbase = __builtin_amdgcn_readlane(wbase, 63);
     int pos = wbase + incl - cnt;
+    int *dumOut = it.dummyOut + lane;
 #pragma unroll
-    for (int p = 0; p <= L; ++p) {
-      if ((hit >> p) & 1ull) it.qout[pos++] = (p < L && p < len) ? tt[p < L ? p : 0] : ci;
+    for (int j = 0; j <= L; ++j) {
+      const bool h = (hit >> j) & 1u;
+      *(h ? &it.qout[pos] : dumOut) = j < L ? tt[j < L ? j : 0] : ci;
+      pos += h ? 1 : 0;
     }
   }
 }
 
-/* LDS words past the edge records: a row's branch-free gathers read up to
- * 64 records past its start */
-constexpr int kLdsSlack = 4 * 64;
+/* rlL word of a core vertex: row length, and a flag for rows that name an
+ * eligible vertex outside the core */
+constexpr int kRowLenMask = 0x3fffffff;
+constexpr int kRowHasOut = 0x40000000;
+
+/* LDS words past the working set (rounding of the aligned sections) */
+constexpr int kLdsSlack = 64;
 
 template <int LT, int SR>
 __global__ __launch_bounds__(LT) void k_propagate_local(PropArgs a, LocalArgs l, int k0,
                                                         int maxIter) {
+  static_assert(SR <= kLaneRowMax, "padded stride above the lane row limit");
+  /* padded rows are SR + 1 records apart: with 2*(SR+1) words (SR a power
+   * of two) the rows of neighbouring core vertices start in different LDS
+   * banks, so a wave's lanes reading position j of their rows do not all
+   * collide in a few banks */
+  constexpr int RS = SR + 1;
   extern __shared__ int4 pool4[];
   int *pool = (int *)pool4;
   __shared__ int sh[SH_COUNT];
   __shared__ int tmp[17];
+  __shared__ int dummyStamp[64], dummyOut[64];
   DevCounters *c = a.ctr;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int P = l.ldsWords;
@@ -401,6 +419,7 @@ __global__ __launch_bounds__(LT) void k_propagate_local(PropArgs a, LocalArgs l,
   bool useCore = true;
   unsigned profU = 0, profE = 0, profC = 0, profA = 0;
   unsigned long long passUpdates = 0; /* tid 0: frontier sizes of this pass */
+  if (tid < 64) dummyStamp[tid] = DLP_INT_MAX;
 
   while (true) {  // one pass = build the LDS working set, iterate, write back
     /* every thread has read the previous pass's sh[] values */
@@ -418,6 +437,8 @@ __global__ __launch_bounds__(LT) void k_propagate_local(PropArgs a, LocalArgs l,
       sh[SH_REBUILD] = 0;
       sh[SH_NOVF] = 0;
       sh[SH_HASLONG] = 0;
+      sh[SH_NHE] = 0;
+      sh[SH_NHI] = 0;
       sh[SH_NLONG0] = 0;
       sh[SH_NLONG1] = 0;
     }
@@ -443,19 +464,45 @@ __global__ __launch_bounds__(LT) void k_propagate_local(PropArgs a, LocalArgs l,
     }
     wg_global_sync();
     const int nCore = sh[SH_NCORE];
-    int *rlL = gidL + nCand;
-    int *rsL = rlL + nCore;
+    int *rlL = gidL + nCand;  // row length
+    int *rsL = rlL + nCore;   // offset of a row longer than SR in the long records
     for (int i = tid; i < nCore; i += LT) {
       const int g = gidL[i];
       const int rl = a.rowLen ? a.rowLen[g] : a.rowEnd[g] - a.rowBeg[g];
       rlL[i] = rl;
-      if (rl > SR) sh[SH_HASLONG] = 1;
+      rsL[i] = rl > SR ? rl : 0;
+      if (rl > kLaneRowMax) sh[SH_HASLONG] = 1;
     }
     __syncthreads();
-    const int E = block_scan_lds<LT>(rlL, rsL, nCore, tmp);  // ends with a barrier
+    const int Elong = block_scan_lds<LT>(rsL, rsL, nCore, tmp);  // ends with a barrier
     const bool hasLong = sh[SH_HASLONG] != 0;
-    const int recOff = (nCand + 8 * nCore + 3) & ~3;  // 16-byte aligned records
-    if ((long long)recOff + 4LL * E + kLdsSlack > P) {  // uniform
+    /* ---- halo: distinct eligible / ineligible neighbours outside the core */
+    for (int i = tid; i < nCore; i += LT) {
+      const int beg = a.rowBeg[gidL[i]], len = rlL[i];
+      for (int p = 0; p < len; ++p) {
+        const int v = a.colInd[beg + p];
+        if (v < 0 || (a.status && a.status[v] == DLP_VERTEX_DELETED)) continue;
+        if (ld_agent(&l.coreGen[v]) == gen) continue;
+        if (atomicMax(&l.haloGen[v], gen) < gen) {
+          const bool elig = a.iterStamp[v] != DLP_INT_MAX;
+          const int idx = atomicAdd(&sh[elig ? SH_NHE : SH_NHI], 1);
+          l.haloIdx[v] = elig ? idx : (idx | 0x40000000);
+        }
+      }
+    }
+    wg_global_sync();
+    const int nHE = sh[SH_NHE], nHI = sh[SH_NHI];
+    const int nullSlot = nCore + nHE + nHI;
+    const int nLab = nullSlot + 1;
+    /* layout after gidL, rlL, rsL */
+    const int offStamp = nCand + 2 * nCore;
+    const int offQ0 = offStamp + nCore, offQ1 = offQ0 + nCore, offLong = offQ1 + nCore;
+    const int offLab0 = offLong + nCore, offLab1 = offLab0 + nLab;
+    const int offHalo = offLab1 + nLab;
+    const int offRecS = (offHalo + nHE + 1) & ~1;                  // int2 aligned
+    const int offRecL = offRecS + 2 * RS * nCore;
+    const long long need = (long long)offRecL + 2LL * Elong + kLdsSlack;
+    if (need > P) {  // uniform
       if (useCore && cc > 0) {
         useCore = false;
         continue;
@@ -463,37 +510,54 @@ __global__ __launch_bounds__(LT) void k_propagate_local(PropArgs a, LocalArgs l,
       reason = DLP_LOCAL_TOO_BIG;
       break;
     }
-    float *lab0L = (float *)(rsL + nCore);
-    float *lab1L = lab0L + nCore;
-    int *stampL = (int *)(lab1L + nCore);
-    int *q0L = stampL + nCore;
-    int *q1L = q0L + nCore;
-    int *longL = q1L + nCore;
-    int4 *rec = (int4 *)(pool + recOff);
-    /* ---- stage core labels, stamps and rows ------------------------------ */
+    int *stampL = pool + offStamp;
+    int *q0L = pool + offQ0;
+    int *q1L = pool + offQ1;
+    int *longL = pool + offLong;
+    float *lab0L = (float *)(pool + offLab0);
+    float *lab1L = (float *)(pool + offLab1);
+    int *haloE = pool + offHalo;
+    int2 *recS = (int2 *)(pool + offRecS);
+    int2 *recL = (int2 *)(pool + offRecL);
+    /* ---- stage labels, stamps and row records ---------------------------- */
     for (int i = tid; i < nCore; i += LT) {
       const int g = gidL[i];
       lab0L[i] = a.lab0[g];
       lab1L[i] = a.lab1[g];
       stampL[i] = (a.iterStamp[g] == DLP_INT_MAX) ? DLP_INT_MAX : -1;
-      const int beg = a.rowBeg[g], len = rlL[i], out = rsL[i];
+      const int beg = a.rowBeg[g], len = rlL[i];
+      const bool shortRow = len <= SR;
+      int2 *out = shortRow ? recS + i * RS : recL + rsL[i];
+      bool hasOut = false;
       for (int p = 0; p < len; ++p) {
         const int v = a.colInd[beg + p];
-        int4 r = make_int4(-1, __float_as_int(a.values[beg + p]), 0, 0);
-        if (v < 0 || (a.status && a.status[v] == DLP_VERTEX_DELETED)) {
-          r.x = -1;
-        } else if (ld_agent(&l.coreGen[v]) == gen) {
-          r.x = ld_agent(&l.coreIdx[v]);
-        } else {
-          const int elig = a.iterStamp[v] != DLP_INT_MAX ? 1 : 0;
-          r.x = -2 - (2 * v + elig);
-          r.z = __float_as_int(a.lab0[v]);
-          r.w = __float_as_int(a.lab1[v]);
+        int2 r = make_int2(nullSlot, 0);
+        if (v >= 0 && !(a.status && a.status[v] == DLP_VERTEX_DELETED)) {
+          r.y = __float_as_int(a.values[beg + p]);
+          if (ld_agent(&l.coreGen[v]) == gen) {
+            r.x = ld_agent(&l.coreIdx[v]);
+          } else {
+            const int h = ld_agent(&l.haloIdx[v]);
+            const int hi = h & 0x3fffffff;
+            r.x = (h & 0x40000000) ? nCore + nHE + hi : nCore + hi;
+            lab0L[r.x] = a.lab0[v];  // the same values from every entry naming v
+            lab1L[r.x] = a.lab1[v];
+            if (!(h & 0x40000000)) {
+              haloE[hi] = v;
+              hasOut = true;
+            }
+          }
         }
-        rec[out + p] = r;
+        out[p] = r;
       }
+      if (hasOut) rlL[i] = len | kRowHasOut;
+      /* padding of a short row; the unused short slots of a long row */
+      for (int p = shortRow ? len : 0; p < SR; ++p) recS[i * RS + p] = make_int2(nullSlot, 0);
     }
-    for (int i = tid; i < kLdsSlack / 4; i += LT) rec[E + i] = make_int4(-1, 0, 0, 0);
+    if (tid == 0) {
+      lab0L[nullSlot] = 0.0f;
+      lab1L[nullSlot] = 0.0f;
+    }
     __syncthreads();
     /* ---- frontier of iteration k ----------------------------------------- */
     const int nq = nCand - cc;
@@ -537,45 +601,49 @@ __global__ __launch_bounds__(LT) void k_propagate_local(PropArgs a, LocalArgs l,
       const float *Fo = kb ? lab1L : lab0L;
       float *Fn = kb ? lab0L : lab1L;
       int *nLongCtr = &sh[SH_NLONG0 + kb];
-      const LocalIter it{rec, stampL, kb ? q0L : q1L, &sh[SH_QC0 + (k + 1) % 3], sh,
-                         k + 1, stampBase + k + 1, stampBase, kb};
+      const LocalIter it{stampL, kb ? q0L : q1L, &sh[SH_QC0 + (k + 1) % 3], sh, haloE,
+                         dummyStamp, dummyOut, nCore, nHE, k + 1, stampBase + k + 1,
+                         stampBase};
 
-      /* short rows: one lane per frontier vertex, the wave's longest row
-       * picks the unrolled width */
+      /* rows up to kLaneRowMax: one lane per frontier vertex */
       for (int base = 0; base < n; base += LT) {  // block-uniform
         if (base + (tid & ~63) >= n) continue;     // wave-uniform: no vertex for this wave
         const int item = base + tid;
         const bool act = item < n;
         const int ci = act ? qin[item] : 0;
-        const int rs = act ? rsL[ci] : 0;
-        const int len = act ? rlL[ci] : 0;
+        const int rl = act ? rlL[ci] : 0;
+        const int len = rl & kRowLenMask;
+        const bool hasOut = (rl & kRowHasOut) != 0;
         const bool shortRow = act && len <= SR;
+        const bool midRow = act && len > SR && len <= kLaneRowMax;
         if (hasLong) {
-          const int slot = dlp_wave_append(nLongCtr, act && !shortRow);
+          const int slot = dlp_wave_append(nLongCtr, act && len > kLaneRowMax);
           if (slot >= 0) longL[slot] = ci;
         }
-        const int lenS = shortRow ? len : 0;
-        if (!__any(lenS > 4)) {
-          local_short_rows<4>(a, l, it, Fo, Fn, shortRow, ci, rs, lenS, profE, profC, profA);
-        } else if (SR <= 8 || !__any(lenS > 8)) {
-          local_short_rows<(SR < 8 ? SR : 8)>(a, l, it, Fo, Fn, shortRow, ci, rs, lenS, profE,
-                                              profC, profA);
-        } else if (SR <= 16 || !__any(lenS > 16)) {
-          local_short_rows<(SR < 16 ? SR : 16)>(a, l, it, Fo, Fn, shortRow, ci, rs, lenS, profE,
-                                                profC, profA);
-        } else {
-          local_short_rows<SR>(a, l, it, Fo, Fn, shortRow, ci, rs, lenS, profE, profC, profA);
+        if (__any(shortRow)) {  // wave-uniform
+          const int2 *row = recS + ci * RS;
+          if (SR > 4 && !__any(shortRow && len > 4))
+            lane_rows<(SR > 4 ? 4 : SR), false>(a, l, it, row, nullSlot, Fo, Fn, shortRow, ci,
+                                                len, hasOut, profE, profC, profA);
+          else
+            lane_rows<SR, false>(a, l, it, row, nullSlot, Fo, Fn, shortRow, ci, len, hasOut,
+                                 profE, profC, profA);
+        }
+        if (SR < kLaneRowMax && __any(midRow)) {  // wave-uniform
+          const int2 *row = recL + (midRow ? rsL[ci] : 0);
+          lane_rows<kLaneRowMax, true>(a, l, it, row, nullSlot, Fo, Fn, midRow, ci,
+                                       midRow ? len : 0, hasOut, profE, profC, profA);
         }
       }
-      /* long rows: one wave per frontier vertex */
+      /* longer rows: one wave per frontier vertex */
       if (hasLong) {  // uniform
         __syncthreads();
         const int nLong = *nLongCtr;
         for (int i = wave; i < nLong; i += LT / 64) {  // wave-uniform
           const int ci = longL[i];
-          const int rs = rsL[ci], len = rlL[ci];
+          const int rs = rsL[ci], len = rlL[ci] & kRowLenMask;
           float W, S;
-          dlp::group_row_sums<64>(lane, len, LocalPos{rec, Fo, rs, kb}, W, S);
+          dlp::group_row_sums<64>(lane, len, LocalPos{recL, Fo, rs}, W, S);
           int changed = 0;
           if (lane == 0) {
             const float oldLabel = Fo[ci];
@@ -589,14 +657,14 @@ __global__ __launch_bounds__(LT) void k_propagate_local(PropArgs a, LocalArgs l,
           if (!changed) continue;  // wave-uniform
           for (int p0 = 0; p0 <= len; p0 += 64) {  // wave-uniform
             const int p = p0 + lane;
-            int t = -1;
-            if (p < len) t = rec[rs + p].x;
+            int t = nullSlot;
+            if (p < len) t = recL[rs + p].x;
             else if (p == len) t = ci;  // the vertex itself
             bool appL = false;
-            if (t >= 0) {
+            if (t < nCore) {
               appL = atomicMax(&stampL[t], it.stampNextL) < it.stampNextL;
-            } else if (t <= -2 && ((-2 - t) & 1)) {
-              profA += mark_outside(a, l, it, (-2 - t) >> 1);
+            } else if (t < nCore + nHE) {
+              profA += mark_outside(a, l, it, haloE[t - nCore]);
             }
             profA += appL ? 1u : 0u;
             const int slot = dlp_wave_append(it.qcOut, appL);
@@ -699,7 +767,7 @@ int dlp_launch_propagate(const PropArgs &a, int k, int groupLanes, int blocks, h
   return DLP_OK;
 }
 
-#define DLP_LOCAL_SHAPES(X) X(0, 1024, 32) X(1, 1024, 16) X(2, 512, 32) X(3, 256, 32) X(4, 1024, 8)
+#define DLP_LOCAL_SHAPES(X) X(0, 1024, 8) X(1, 1024, 16) X(2, 512, 8) X(3, 256, 8) X(4, 1024, 4)
 
 int dlp_local_shape_count() { return kNumLocalShapes; }
 
@@ -728,7 +796,7 @@ int dlp_launch_propagate_local(const PropArgs &a, const LocalArgs &l, int k, int
   switch (l.shape) {
 #define DLP_LAUNCH_LOCAL(I, T, R)                                                       \
   case I:                                                                              \
-    static_assert(kLocalShapes[I].threads == T && kLocalShapes[I].shortRow == R, "");  \
+    static_assert(kLocalShapes[I].threads == T && kLocalShapes[I].stride == R, "");  \
     hipLaunchKernelGGL((k_propagate_local<T, R>), dim3(1), dim3(T), lds, s, a, l, k, maxIter); \
     break;
     DLP_LOCAL_SHAPES(DLP_LAUNCH_LOCAL)
