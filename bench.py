@@ -66,6 +66,40 @@ def engine_rooflines(prof: dict, updates: int) -> dict:
     return out
 
 
+def measured_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` on this workload, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (None when absent)."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{kernel}_c2.txt")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("hbm_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
+def copy_calibration(mib: int) -> dict:
+    """One device-to-device copy of `mib` MiB with the library's own copy
+    kernel (k_copy_vec16): reads and writes exactly that many bytes."""
+    import ctypes as C
+    from labelpropcuda_amd import _lib
+    lib = _lib.load()
+    n = mib << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    _lib.check(lib.dlp_device_malloc(C.byref(a), n))
+    _lib.check(lib.dlp_device_malloc(C.byref(b), n))
+    try:
+        t0 = time.perf_counter()
+        _lib.check(lib.dlp_copy_device(b, a, n))
+        dt = time.perf_counter() - t0
+    finally:
+        lib.dlp_device_free(a)
+        lib.dlp_device_free(b)
+    return {"bytes": n, "wall_s": dt}
+
+
 def cpu_baseline(inputs, budget_s: float = 40.0):
     """The CPU oracle (single thread) on the same stream, batch by batch,
     stopping once the time budget is spent; returns updates/s over the
@@ -100,6 +134,9 @@ def main() -> int:
     ap.add_argument("--local-shape", type=int, default=0,
                     help="work-group shape of the LDS-resident kernel (1-based)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="after the run, copy this many MiB device to device with the "
+                         "library's copy kernel (reference point for HBM counters)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -177,7 +214,8 @@ def main() -> int:
         "batch_latency_ms": {"p50": float(np.percentile(lat, 50)),
                              "p99": float(np.percentile(lat, 99))},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(dominant) if args.nodes == 500000 else None,
                      "kernel": dominant,
                      "avg_launch_us": dom["avg_launch_us"],
                      "algorithmic_bytes_per_launch": dom["algorithmic_bytes_per_launch"]},
@@ -189,6 +227,8 @@ def main() -> int:
         out["cpu_baseline"] = {"value": v, "unit": "updates/s", "cores": 1, "kind": "port",
                                "sample": f"first {done} of {inputs.numBatches} batches of the "
                                          f"same stream, {t:.1f} s, CPU oracle (single thread)"}
+    if args.calibrate_copy_mib > 0:
+        out["copy_calibration"] = copy_calibration(args.calibrate_copy_mib)
     print(json.dumps(out))
     return 0
 

@@ -331,6 +331,9 @@ int dlp_device_malloc(void **ptr, size_t bytes);
 int dlp_device_free(void *ptr);
 int dlp_copy_to_device(void *dst, const void *src, size_t bytes);
 int dlp_copy_to_host(void *dst, const void *src, size_t bytes);
+/* device -> device with the library's own copy kernel (16-byte vectors when
+ * both pointers are 16-byte aligned); synchronous */
+int dlp_copy_device(void *dst, const void *src, size_t bytes);
 
 #ifdef __cplusplus
 }

@@ -185,6 +185,7 @@ SIGNATURES = {
     "dlp_inputs_graph": (C.c_int, [_VP, C.POINTER(GraphView), C.POINTER(f32p)]),
     "dlp_inputs_batches": (C.c_int, [_VP, C.POINTER(BatchView)]),
     "dlp_inputs_free": (None, [_VP]),
+    "dlp_copy_device": (C.c_int, [_VP, _VP, C.c_size_t]),
     "dlp_device_malloc": (C.c_int, [C.POINTER(_VP), C.c_size_t]),
     "dlp_device_free": (C.c_int, [_VP]),
     "dlp_copy_to_device": (C.c_int, [_VP, _VP, C.c_size_t]),

@@ -902,3 +902,42 @@ extern "C" int dlp_copy_to_host(void *dst, const void *src, size_t bytes) {
   DLP_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
   return DLP_OK;
 }
+
+namespace {
+__global__ void k_copy_vec16(uint4 *dst, const uint4 *src, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+__global__ void k_copy_bytes(unsigned char *dst, const unsigned char *src, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+}  // namespace
+
+extern "C" int dlp_copy_device(void *dst, const void *src, size_t bytes) {
+  if (bytes == 0) return DLP_OK;
+  if (!dst || !src) {
+    dlp_set_error_("dlp_copy_device: NULL argument");
+    return DLP_ERR_INVALID;
+  }
+  size_t done = 0;
+  if ((((uintptr_t)dst | (uintptr_t)src) & 15) == 0 && bytes >= 16) {
+    const size_t n = bytes / 16;
+    const unsigned grid = (unsigned)std::min<size_t>((n + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_copy_vec16, dim3(grid), dim3(256), 0, 0, (uint4 *)dst,
+                       (const uint4 *)src, n);
+    DLP_LAUNCHED("k_copy_vec16");
+    done = n * 16;
+  }
+  if (done < bytes) {
+    const size_t n = bytes - done;
+    const unsigned grid = (unsigned)std::min<size_t>((n + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_copy_bytes, dim3(grid), dim3(256), 0, 0, (unsigned char *)dst + done,
+                       (const unsigned char *)src + done, n);
+    DLP_LAUNCHED("k_copy_bytes");
+  }
+  DLP_HIP(hipDeviceSynchronize());
+  return DLP_OK;
+}

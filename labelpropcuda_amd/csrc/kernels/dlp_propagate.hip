@@ -174,7 +174,10 @@ __global__ __launch_bounds__(kPT) void k_propagate(PropArgs a, int k) {
   __syncthreads();
   stage_flush<kStageQ>(stageQ, stageCount[0], outCount, out, &stageBase[0]);
   if (a.touched) stage_flush<kStageT>(stageT, stageCount[1], &c->touched, a.touched, &stageBase[1]);
-  if (a.profile) {  // one atomic per counter per wave
+  if (a.profile) {  // LDS sums per block, then one global atomic per counter per block
+    __shared__ unsigned long long blockProf[3];
+    if (threadIdx.x < 3) blockProf[threadIdx.x] = 0ull;
+    __syncthreads();
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) {
       profE += __shfl_xor(profE, off, 64);
@@ -182,9 +185,15 @@ __global__ __launch_bounds__(kPT) void k_propagate(PropArgs a, int k) {
       profA += __shfl_xor(profA, off, 64);
     }
     if (lane == 0) {
-      if (profE) atomicAdd(&c->profEdges, (unsigned long long)profE);
-      if (profC) atomicAdd(&c->profChangedEdges, (unsigned long long)profC);
-      if (profA) atomicAdd(&c->profAppended, (unsigned long long)profA);
+      atomicAdd(&blockProf[0], (unsigned long long)profE);
+      atomicAdd(&blockProf[1], (unsigned long long)profC);
+      atomicAdd(&blockProf[2], (unsigned long long)profA);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      if (blockProf[0]) atomicAdd(&c->profEdges, blockProf[0]);
+      if (blockProf[1]) atomicAdd(&c->profChangedEdges, blockProf[1]);
+      if (blockProf[2]) atomicAdd(&c->profAppended, blockProf[2]);
     }
   }
 }
