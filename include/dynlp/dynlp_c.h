@@ -324,6 +324,45 @@ int dlp_inputs_batches(const dlp_inputs *in, dlp_batch_view *batches);
 void dlp_inputs_free(dlp_inputs *in);
 
 /* ====================================================================== */
+/* 5b. Many independent graphs: one propagation launch per batch index    */
+/*    (no counterpart in the reference, which updates one graph at a     */
+/*    time; its per-graph semantics are runDynLP's, dynLP.cu:175-587)     */
+/* ====================================================================== */
+typedef struct dlp_multi dlp_multi;
+
+typedef struct dlp_multi_tuning {
+  int localShape;       /* work-group shape of the propagation launch,
+                           1-based as dlp_tuning.localShape (default 3)  */
+  int thresholdPercent; /* scales each graph's LDS-pass frontier
+                           threshold (default 100); -1 = no LDS passes   */
+  int backoff;          /* HBM iterations after an LDS overflow (8)      */
+  int useGraphs;        /* 1 = capture each batch in a HIP graph and
+                           replay it (default), 2 = enqueue every time   */
+} dlp_multi_tuning;
+
+/* One context per graph (arrays of numGraphs configs, graphs, true-label
+ * pointers and batch views), all on one internal stream. */
+int dlp_multi_create(dlp_multi **out, int numGraphs, const dlp_config *cfgs,
+                     const dlp_graph_view *graphs, const float *const *trueLabels,
+                     const dlp_batch_view *batches);
+int dlp_multi_destroy(dlp_multi *m);
+/* fields left 0 keep their value; resets every graph's state */
+int dlp_multi_set_tuning(dlp_multi *m, const dlp_multi_tuning *t);
+int dlp_multi_get_tuning(const dlp_multi *m, dlp_multi_tuning *t);
+int dlp_multi_reset(dlp_multi *m);
+/* batch b of every graph that has one; batches run in order from a reset */
+int dlp_multi_run_batch(dlp_multi *m, int b);
+/* reset, then every batch */
+int dlp_multi_run_all(dlp_multi *m);
+/* waits for the stream and checks every graph's device error flags */
+int dlp_multi_synchronize(dlp_multi *m);
+int dlp_multi_read_labels(dlp_multi *m, int graph, float *labels);
+/* per-graph record of batch b; timeMs is the all-graph batch time */
+int dlp_multi_batch_stats(dlp_multi *m, int graph, int b, dlp_batch_stats *out);
+int dlp_multi_batch_time(dlp_multi *m, int b, float *ms);
+int dlp_multi_device_bytes(const dlp_multi *m, int64_t *bytes);
+
+/* ====================================================================== */
 /* 5. Device-memory plumbing for callers without a GPU runtime of their   */
 /*    own (the reference calls cudaMalloc/cudaMemcpy directly)            */
 /* ====================================================================== */

@@ -5,7 +5,8 @@ API of include/dynlp/dynLP.h.  This package is the Python plumbing over that
 C-ABI used by the tests and the benchmark.
 """
 from ._lib import DynLPError, device_count, load  # noqa: F401
-from .dynlp import Context, Inputs, TraceBuffers, generate, make_config, run_dynlp  # noqa: F401
+from .dynlp import (Context, Inputs, Multi, TraceBuffers, batched_workload,  # noqa: F401
+                    generate, make_config, run_dynlp)
 
-__all__ = ["Context", "Inputs", "TraceBuffers", "generate", "make_config", "run_dynlp",
-           "DynLPError", "device_count", "load"]
+__all__ = ["Context", "Inputs", "Multi", "TraceBuffers", "batched_workload", "generate",
+           "make_config", "run_dynlp", "DynLPError", "device_count", "load"]

@@ -122,6 +122,15 @@ class Trace(C.Structure):
     ]
 
 
+class MultiTuning(C.Structure):
+    _fields_ = [
+        ("localShape", C.c_int),
+        ("thresholdPercent", C.c_int),
+        ("backoff", C.c_int),
+        ("useGraphs", C.c_int),
+    ]
+
+
 class PropProfile(C.Structure):
     _fields_ = [
         ("launches", C.c_int64),
@@ -185,6 +194,20 @@ SIGNATURES = {
     "dlp_inputs_graph": (C.c_int, [_VP, C.POINTER(GraphView), C.POINTER(f32p)]),
     "dlp_inputs_batches": (C.c_int, [_VP, C.POINTER(BatchView)]),
     "dlp_inputs_free": (None, [_VP]),
+    "dlp_multi_create": (C.c_int, [C.POINTER(_VP), C.c_int, C.POINTER(Config),
+                                   C.POINTER(GraphView), C.POINTER(C.POINTER(C.c_float)),
+                                   C.POINTER(BatchView)]),
+    "dlp_multi_destroy": (C.c_int, [_VP]),
+    "dlp_multi_set_tuning": (C.c_int, [_VP, C.POINTER(MultiTuning)]),
+    "dlp_multi_get_tuning": (C.c_int, [_VP, C.POINTER(MultiTuning)]),
+    "dlp_multi_reset": (C.c_int, [_VP]),
+    "dlp_multi_run_batch": (C.c_int, [_VP, C.c_int]),
+    "dlp_multi_run_all": (C.c_int, [_VP]),
+    "dlp_multi_synchronize": (C.c_int, [_VP]),
+    "dlp_multi_read_labels": (C.c_int, [_VP, C.c_int, C.POINTER(C.c_float)]),
+    "dlp_multi_batch_stats": (C.c_int, [_VP, C.c_int, C.c_int, C.POINTER(BatchStats)]),
+    "dlp_multi_batch_time": (C.c_int, [_VP, C.c_int, C.POINTER(C.c_float)]),
+    "dlp_multi_device_bytes": (C.c_int, [_VP, C.POINTER(C.c_int64)]),
     "dlp_copy_device": (C.c_int, [_VP, _VP, C.c_size_t]),
     "dlp_device_malloc": (C.c_int, [C.POINTER(_VP), C.c_size_t]),
     "dlp_device_free": (C.c_int, [_VP]),

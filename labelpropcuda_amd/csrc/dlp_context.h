@@ -1,0 +1,103 @@
+/*
+ * dlp_context.h -- the device context of one evolving graph (internal).
+ */
+#ifndef DLP_CONTEXT_H
+#define DLP_CONTEXT_H
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "dlp_kernels.h"
+#include "dynlp/dynlp_c.h"
+
+/* ------------------------------------------------------------------------ */
+/* context                                                                  */
+/* ------------------------------------------------------------------------ */
+struct dlp_context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool ownStream = false;
+  dlp_config cfg{};
+  float tau = 0.5f;
+  int N = 0, B = 0;
+  long long nnzFull = 0;
+  std::vector<int64_t> insOff, gtOff, delOff, eOff;
+  std::vector<void *> allocs;
+  int64_t deviceBytes = 0;
+  DState d{};
+  int *dIns = nullptr, *dGtIds = nullptr, *dDel = nullptr, *dESrc = nullptr, *dEDst = nullptr;
+  float *dGtLab = nullptr, *dEW = nullptr;
+  int *iterSizes = nullptr;
+  int iterSizesCap = 0;
+  DevCounters *pinCtr = nullptr;
+  dlp_tuning tuning{8, 512, 8, 0, 0, 1}; /* local threshold and LDS set at creation */
+  LocalArgs local{};
+  int localMaxBytes = 0;
+  int nextBatch = 0;
+  int markSeq = 0;
+  int lastCCBatch = -1;
+  std::vector<hipEvent_t> evStart, evEnd;
+  std::vector<DevBatchRecord> records;
+  std::vector<char> ran;
+  dlp_trace *trace = nullptr;
+  /* profiling: event-timed propagation launches */
+  int profiling = 0;
+  PropTiming timing;
+
+  template <class T>
+  int alloc(T **p, long long count) {
+    size_t bytes = (size_t)std::max<long long>(count, 1) * sizeof(T);
+    void *q = nullptr;
+    hipError_t e = hipMalloc(&q, bytes);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      dlp_set_errorf_("hipMalloc of %zu bytes failed (%s)", bytes, hipGetErrorString(e));
+      return DLP_ERR_NOMEM;
+    }
+    allocs.push_back(q);
+    deviceBytes += (int64_t)bytes;
+    *p = (T *)q;
+    return DLP_OK;
+  }
+  template <class T>
+  int upload(T **p, const T *host, long long count) {
+    int rc = alloc(p, count);
+    if (rc != DLP_OK) return rc;
+    if (count > 0) DLP_HIP(hipMemcpy(*p, host, (size_t)count * sizeof(T), hipMemcpyHostToDevice));
+    return DLP_OK;
+  }
+  void release() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    for (void *p : allocs) (void)hipFree(p);
+    allocs.clear();
+    for (hipEvent_t e : timing.ev) (void)hipEventDestroy(e);
+    timing.ev.clear();
+    for (hipEvent_t e : evStart) (void)hipEventDestroy(e);
+    for (hipEvent_t e : evEnd) (void)hipEventDestroy(e);
+    evStart.clear();
+    evEnd.clear();
+    if (pinCtr) (void)hipHostFree(pinCtr);
+    pinCtr = nullptr;
+    if (ownStream && stream) (void)hipStreamDestroy(stream);
+    stream = nullptr;
+  }
+};
+
+
+/* the batch pipeline of a context, in three phases (same stream) */
+DBatch dlp_ctx_make_batch(const dlp_context *c, int b);
+/* status update .. first frontier; `markTag` receives the batch's mark tag */
+int dlp_ctx_batch_pre(dlp_context *c, int b, const DBatch &bt, int *markTag);
+/* the arguments of the context's propagation kernels */
+PropArgs dlp_ctx_prop_args(const dlp_context *c);
+/* GT enforcement, accuracy, batch record; marks the batch as run */
+int dlp_ctx_batch_post(dlp_context *c, int b, const DBatch &bt);
+/* the host bookkeeping batch b's pre and post phases perform (mark tag
+ * sequence, last batch with a CC step, run flags), for a batch replayed
+ * from a captured graph */
+void dlp_ctx_advance_host(dlp_context *c, int b, const DBatch &bt);
+/* the batch record of batch b (no timing); synchronises the stream */
+int dlp_ctx_fill_stats(dlp_context *c, int b, dlp_batch_stats *out);
+
+#endif /* DLP_CONTEXT_H */

@@ -75,6 +75,24 @@ int dlp_local_lds_bytes(int *bytes);
  * padded to 8 records) */
 int dlp_local_shape_count();
 
+/* one graph of a multi-graph propagation launch */
+struct GraphProp {
+  PropArgs a;
+  LocalArgs l;      /* ldsWords == 0: HBM iterations only             */
+  int maxIter;
+  int threshold;    /* frontier size at or below which LDS passes run */
+  int backoff;      /* HBM iterations after an LDS overflow           */
+};
+
+/* Runs the propagation of every graph in `graphs` (device array, count
+ * numGraphs) to completion in one launch, one work-group per graph, with
+ * work-group shape `shape` (as LocalArgs::shape) and dynamic LDS of
+ * ldsWords words (every graph's l.ldsWords must be 0 or at most that).
+ * Each graph's queue, counters and labels must be set up as for
+ * dlp_run_propagation. */
+int dlp_launch_propagate_multi(const GraphProp *graphs, int numGraphs, int shape, int ldsWords,
+                               hipStream_t s);
+
 /* HIP-event timing of propagation launches (optional) */
 struct PropTiming {
   std::vector<hipEvent_t> ev; /* pairs, reused                          */

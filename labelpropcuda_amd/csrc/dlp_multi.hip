@@ -1,0 +1,396 @@
+/*
+ * dlp_multi.hip -- many independent evolving graphs updated together.
+ *
+ * One context per graph (its own device state, all on one stream).  A batch
+ * index b runs, for every graph that has a batch b, the pipeline up to the
+ * first frontier, then ONE propagation launch for all of them (one
+ * work-group per graph, k_propagate_multi), then the per-graph batch end.
+ * The propagation needs no host decision, so the whole batch -- pipelines,
+ * the propagation launch and the batch ends -- is captured once into a HIP
+ * graph and replayed on later steps; the state reset of all graphs is a
+ * graph of its own.  The reference updates one graph at a time with a host
+ * round trip per propagation iteration (dynLP.cu:175-587,
+ * iterativePropagation.cu:274-327).
+ */
+#include <algorithm>
+#include <vector>
+
+#include "dlp_context.h"
+
+struct dlp_multi {
+  int G = 0, maxB = 0;
+  hipStream_t stream = nullptr;
+  std::vector<dlp_context *> ctx;
+  int ldsWords = 0;
+  int shape = 2;        /* LocalArgs::shape of the propagation launch */
+  int backoff = 8;
+  int thresholdPct = 100;
+  bool useGraphs = true;
+  std::vector<GraphProp *> dProps; /* per batch: graphs that have it */
+  std::vector<int> counts;
+  std::vector<hipGraphExec_t> exec;
+  hipGraphExec_t resetExec = nullptr;
+  std::vector<hipEvent_t> evStart, evEnd;
+  std::vector<char> ran;
+  int nextBatch = 0;
+
+  void drop_graphs() {
+    for (hipGraphExec_t &e : exec)
+      if (e) {
+        (void)hipGraphExecDestroy(e);
+        e = nullptr;
+      }
+    if (resetExec) (void)hipGraphExecDestroy(resetExec);
+    resetExec = nullptr;
+  }
+  void release() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    drop_graphs();
+    for (GraphProp *p : dProps)
+      if (p) (void)hipFree(p);
+    dProps.clear();
+    for (dlp_context *c : ctx) dlp_context_destroy(c);
+    ctx.clear();
+    for (hipEvent_t e : evStart) (void)hipEventDestroy(e);
+    for (hipEvent_t e : evEnd) (void)hipEventDestroy(e);
+    evStart.clear();
+    evEnd.clear();
+    if (stream) (void)hipStreamDestroy(stream);
+    stream = nullptr;
+  }
+};
+
+namespace {
+
+/* per-graph descriptors of batch b's propagation launch */
+int upload_props(dlp_multi *m) {
+  for (int b = 0; b < m->maxB; ++b) {
+    std::vector<GraphProp> h;
+    for (dlp_context *c : m->ctx) {
+      if (b >= c->B) continue;
+      GraphProp g{};
+      g.a = dlp_ctx_prop_args(c);
+      g.l = c->local;
+      g.l.ldsWords = m->ldsWords;
+      g.maxIter = c->cfg.maxIterations;
+      const long long th = (long long)c->tuning.localThreshold * m->thresholdPct / 100;
+      g.threshold = m->thresholdPct < 0 ? 0 : (int)std::max<long long>(1, std::min<long long>(th, 1 << 30));
+      if (c->N >= (1 << 30)) g.threshold = 0;  // outside-vertex encoding limit
+      g.backoff = m->backoff;
+      h.push_back(g);
+    }
+    m->counts[b] = (int)h.size();
+    if (!m->dProps[b]) {
+      void *p = nullptr;
+      DLP_HIP(hipMalloc(&p, sizeof(GraphProp) * std::max<size_t>(h.size(), 1)));
+      m->dProps[b] = (GraphProp *)p;
+    }
+    if (!h.empty())
+      DLP_HIP(hipMemcpy(m->dProps[b], h.data(), sizeof(GraphProp) * h.size(),
+                        hipMemcpyHostToDevice));
+  }
+  return DLP_OK;
+}
+
+/* enqueue batch b of every graph on the shared stream */
+int enqueue_batch(dlp_multi *m, int b) {
+  int rc;
+  std::vector<DBatch> bts(m->G);
+  for (int g = 0; g < m->G; ++g) {
+    dlp_context *c = m->ctx[g];
+    if (b >= c->B) continue;
+    bts[g] = dlp_ctx_make_batch(c, b);
+    int tag = 0;
+    if ((rc = dlp_ctx_batch_pre(c, b, bts[g], &tag)) != DLP_OK) return rc;
+  }
+  if ((rc = dlp_launch_propagate_multi(m->dProps[b], m->counts[b], m->shape, m->ldsWords,
+                                       m->stream)) != DLP_OK)
+    return rc;
+  for (int g = 0; g < m->G; ++g) {
+    dlp_context *c = m->ctx[g];
+    if (b >= c->B) continue;
+    if ((rc = dlp_ctx_batch_post(c, b, bts[g])) != DLP_OK) return rc;
+  }
+  return DLP_OK;
+}
+
+int enqueue_reset(dlp_multi *m) {
+  for (dlp_context *c : m->ctx) {
+    const int rc = dlp_reset_state(c->d, m->stream);
+    if (rc != DLP_OK) return rc;
+  }
+  return DLP_OK;
+}
+
+/* capture what `fn` enqueues into an executable graph */
+template <class Fn>
+int capture(dlp_multi *m, hipGraphExec_t *out, Fn fn) {
+  DLP_HIP(hipStreamBeginCapture(m->stream, hipStreamCaptureModeThreadLocal));
+  const int rc = fn();
+  hipGraph_t graph = nullptr;
+  const hipError_t e = hipStreamEndCapture(m->stream, &graph);
+  if (rc != DLP_OK) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return rc;
+  }
+  if (e != hipSuccess) return dlp_hip_fail_(e, "hipStreamEndCapture");
+  const hipError_t ie = hipGraphInstantiate(out, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (ie != hipSuccess) return dlp_hip_fail_(ie, "hipGraphInstantiate");
+  return DLP_OK;
+}
+
+void host_reset(dlp_multi *m) {
+  for (dlp_context *c : m->ctx) {
+    c->nextBatch = 0;
+    c->lastCCBatch = -1;
+    std::fill(c->ran.begin(), c->ran.end(), 0);
+  }
+  m->nextBatch = 0;
+  std::fill(m->ran.begin(), m->ran.end(), 0);
+}
+
+}  // namespace
+
+extern "C" int dlp_multi_create(dlp_multi **out, int numGraphs, const dlp_config *cfgs,
+                                const dlp_graph_view *graphs, const float *const *trueLabels,
+                                const dlp_batch_view *batches) {
+  if (!out || numGraphs <= 0 || !cfgs || !graphs || !trueLabels || !batches) {
+    dlp_set_error_("dlp_multi_create: bad argument");
+    return DLP_ERR_INVALID;
+  }
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+    (void)hipGetLastError();
+    dlp_set_error_("no GPU visible to HIP");
+    return DLP_ERR_NODEVICE;
+  }
+  dlp_multi *m = new dlp_multi();
+  m->G = numGraphs;
+  int rc = DLP_OK;
+  int ldsBytes = 0;
+#define MTRY(x)           \
+  do {                    \
+    rc = (x);             \
+    if (rc != DLP_OK) {   \
+      m->release();       \
+      delete m;           \
+      return rc;          \
+    }                     \
+  } while (0)
+  if (hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete m;
+    dlp_set_error_("hipStreamCreate failed");
+    return DLP_ERR_HIP;
+  }
+  MTRY(dlp_local_lds_bytes(&ldsBytes));
+  m->ldsWords = ldsBytes / 4;
+  for (int g = 0; g < numGraphs; ++g) {
+    dlp_context *c = nullptr;
+    rc = dlp_context_create(&c, &cfgs[g], &graphs[g], trueLabels[g], &batches[g], m->stream);
+    if (rc != DLP_OK) {
+      const std::string msg = dlp_last_error();
+      m->release();
+      delete m;
+      dlp_set_errorf_("graph %d: %s", g, msg.c_str());
+      return rc;
+    }
+    m->ctx.push_back(c);
+    m->maxB = std::max(m->maxB, c->B);
+  }
+  m->dProps.assign(m->maxB, nullptr);
+  m->counts.assign(m->maxB, 0);
+  m->exec.assign(m->maxB, nullptr);
+  m->ran.assign(m->maxB, 0);
+  m->evStart.resize(m->maxB);
+  m->evEnd.resize(m->maxB);
+  for (int b = 0; b < m->maxB; ++b) {
+    if (hipEventCreate(&m->evStart[b]) != hipSuccess ||
+        hipEventCreate(&m->evEnd[b]) != hipSuccess) {
+      m->release();
+      delete m;
+      dlp_set_error_("hipEventCreate failed");
+      return DLP_ERR_HIP;
+    }
+  }
+  MTRY(upload_props(m));
+#undef MTRY
+  host_reset(m);
+  *out = m;
+  return DLP_OK;
+}
+
+extern "C" int dlp_multi_destroy(dlp_multi *m) {
+  if (!m) return DLP_OK;
+  m->release();
+  delete m;
+  return DLP_OK;
+}
+
+extern "C" int dlp_multi_set_tuning(dlp_multi *m, const dlp_multi_tuning *t) {
+  if (!m || !t) {
+    dlp_set_error_("dlp_multi_set_tuning: NULL argument");
+    return DLP_ERR_INVALID;
+  }
+  if (t->localShape && (t->localShape < 1 || t->localShape > dlp_local_shape_count())) {
+    dlp_set_errorf_("localShape must be in [1, %d]", dlp_local_shape_count());
+    return DLP_ERR_INVALID;
+  }
+  if (t->thresholdPercent < -1 || t->backoff < 0 || t->useGraphs < 0 || t->useGraphs > 2) {
+    dlp_set_error_("dlp_multi_set_tuning: thresholdPercent < -1, negative backoff or "
+                   "useGraphs not in {0,1,2}");
+    return DLP_ERR_INVALID;
+  }
+  DLP_HIP(hipStreamSynchronize(m->stream));
+  if (t->localShape) m->shape = t->localShape - 1;
+  if (t->thresholdPercent) m->thresholdPct = t->thresholdPercent;
+  if (t->backoff) m->backoff = t->backoff;
+  if (t->useGraphs) m->useGraphs = t->useGraphs == 1;
+  m->drop_graphs();  // captured launches embed the shape
+  m->exec.assign(m->maxB, nullptr);
+  const int rc = upload_props(m);
+  if (rc != DLP_OK) return rc;
+  host_reset(m);
+  for (dlp_context *c : m->ctx) dlp_reset_state(c->d, m->stream);
+  return DLP_OK;
+}
+
+extern "C" int dlp_multi_get_tuning(const dlp_multi *m, dlp_multi_tuning *t) {
+  if (!m || !t) {
+    dlp_set_error_("dlp_multi_get_tuning: NULL argument");
+    return DLP_ERR_INVALID;
+  }
+  t->localShape = m->shape + 1;
+  t->thresholdPercent = m->thresholdPct;
+  t->backoff = m->backoff;
+  t->useGraphs = m->useGraphs ? 1 : 2;
+  return DLP_OK;
+}
+
+extern "C" int dlp_multi_reset(dlp_multi *m) {
+  if (!m) {
+    dlp_set_error_("dlp_multi_reset: NULL handle");
+    return DLP_ERR_STATE;
+  }
+  int rc;
+  if (m->useGraphs) {
+    if (!m->resetExec &&
+        (rc = capture(m, &m->resetExec, [&] { return enqueue_reset(m); })) != DLP_OK)
+      return rc;
+    DLP_HIP(hipGraphLaunch(m->resetExec, m->stream));
+  } else if ((rc = enqueue_reset(m)) != DLP_OK) {
+    return rc;
+  }
+  host_reset(m);
+  return DLP_OK;
+}
+
+extern "C" int dlp_multi_run_batch(dlp_multi *m, int b) {
+  if (!m) {
+    dlp_set_error_("dlp_multi_run_batch: NULL handle");
+    return DLP_ERR_STATE;
+  }
+  if (b != m->nextBatch || b < 0 || b >= m->maxB) {
+    dlp_set_errorf_("batch %d requested, next batch is %d (reset to start over)", b,
+                    m->nextBatch);
+    return DLP_ERR_STATE;
+  }
+  int rc;
+  bool justCaptured = false;
+  if (m->useGraphs && !m->exec[b]) {
+    /* the pipelines read host bookkeeping (mark tags, last CC batch) that
+     * evolves the same way on every run from a reset, so the capture of the
+     * first run is valid for every later one */
+    if ((rc = capture(m, &m->exec[b], [&] { return enqueue_batch(m, b); })) != DLP_OK)
+      return rc;
+    justCaptured = true;  // the capture ran the host bookkeeping already
+  }
+  DLP_HIP(hipEventRecord(m->evStart[b], m->stream));
+  if (m->useGraphs) {
+    DLP_HIP(hipGraphLaunch(m->exec[b], m->stream));
+    if (!justCaptured)
+      for (dlp_context *c : m->ctx)
+        if (b < c->B) dlp_ctx_advance_host(c, b, dlp_ctx_make_batch(c, b));
+  } else if ((rc = enqueue_batch(m, b)) != DLP_OK) {
+    return rc;
+  }
+  DLP_HIP(hipEventRecord(m->evEnd[b], m->stream));
+  m->ran[b] = 1;
+  m->nextBatch = b + 1;
+  return DLP_OK;
+}
+
+extern "C" int dlp_multi_run_all(dlp_multi *m) {
+  int rc = dlp_multi_reset(m);
+  if (rc != DLP_OK) return rc;
+  for (int b = 0; b < m->maxB; ++b)
+    if ((rc = dlp_multi_run_batch(m, b)) != DLP_OK) return rc;
+  return DLP_OK;
+}
+
+extern "C" int dlp_multi_synchronize(dlp_multi *m) {
+  if (!m) {
+    dlp_set_error_("dlp_multi_synchronize: NULL handle");
+    return DLP_ERR_STATE;
+  }
+  DLP_HIP(hipStreamSynchronize(m->stream));
+  for (dlp_context *c : m->ctx) {
+    DevCounters ctr;
+    DLP_HIP(hipMemcpy(&ctr, c->d.ctr, sizeof(ctr), hipMemcpyDeviceToHost));
+    if (ctr.errorFlag) {
+      dlp_set_errorf_("device kernels reported a violated precondition (flags 0x%x)",
+                      ctr.errorFlag);
+      return DLP_ERR_INTERNAL;
+    }
+  }
+  return DLP_OK;
+}
+
+extern "C" int dlp_multi_read_labels(dlp_multi *m, int g, float *labels) {
+  if (!m || g < 0 || g >= m->G) {
+    dlp_set_error_("dlp_multi_read_labels: bad argument");
+    return DLP_ERR_INVALID;
+  }
+  return dlp_context_read_labels(m->ctx[g], labels);
+}
+
+extern "C" int dlp_multi_batch_stats(dlp_multi *m, int g, int b, dlp_batch_stats *out) {
+  if (!m || g < 0 || g >= m->G || !out) {
+    dlp_set_error_("dlp_multi_batch_stats: bad argument");
+    return DLP_ERR_INVALID;
+  }
+  dlp_context *c = m->ctx[g];
+  if (b < 0 || b >= c->B || !c->ran[b]) {
+    dlp_set_errorf_("graph %d: batch %d has not run since the last reset", g, b);
+    return DLP_ERR_STATE;
+  }
+  int rc = dlp_ctx_fill_stats(c, b, out);
+  if (rc != DLP_OK) return rc;
+  /* the graph's batch ran inside the all-graph batch b */
+  float ms = 0.0f;
+  DLP_HIP(hipEventElapsedTime(&ms, m->evStart[b], m->evEnd[b]));
+  out->timeMs = ms;
+  return DLP_OK;
+}
+
+extern "C" int dlp_multi_batch_time(dlp_multi *m, int b, float *ms) {
+  if (!m || !ms || b < 0 || b >= m->maxB || !m->ran[b]) {
+    dlp_set_error_("dlp_multi_batch_time: bad argument or batch not run");
+    return DLP_ERR_INVALID;
+  }
+  DLP_HIP(hipEventSynchronize(m->evEnd[b]));
+  DLP_HIP(hipEventElapsedTime(ms, m->evStart[b], m->evEnd[b]));
+  return DLP_OK;
+}
+
+extern "C" int dlp_multi_device_bytes(const dlp_multi *m, int64_t *bytes) {
+  if (!m || !bytes) {
+    dlp_set_error_("dlp_multi_device_bytes: NULL argument");
+    return DLP_ERR_INVALID;
+  }
+  int64_t t = 0;
+  for (dlp_context *c : m->ctx) t += c->deviceBytes;
+  *bytes = t + (int64_t)sizeof(GraphProp) * m->G * m->maxB;
+  return DLP_OK;
+}

@@ -67,78 +67,7 @@ static int need_device() {
   return DLP_OK;
 }
 
-/* ------------------------------------------------------------------------ */
-/* context                                                                  */
-/* ------------------------------------------------------------------------ */
-struct dlp_context {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  bool ownStream = false;
-  dlp_config cfg{};
-  float tau = 0.5f;
-  int N = 0, B = 0;
-  long long nnzFull = 0;
-  std::vector<int64_t> insOff, gtOff, delOff, eOff;
-  std::vector<void *> allocs;
-  int64_t deviceBytes = 0;
-  DState d{};
-  int *dIns = nullptr, *dGtIds = nullptr, *dDel = nullptr, *dESrc = nullptr, *dEDst = nullptr;
-  float *dGtLab = nullptr, *dEW = nullptr;
-  int *iterSizes = nullptr;
-  int iterSizesCap = 0;
-  DevCounters *pinCtr = nullptr;
-  dlp_tuning tuning{8, 512, 8, 0, 0, 1}; /* local threshold and LDS set at creation */
-  LocalArgs local{};
-  int localMaxBytes = 0;
-  int nextBatch = 0;
-  int markSeq = 0;
-  int lastCCBatch = -1;
-  std::vector<hipEvent_t> evStart, evEnd;
-  std::vector<DevBatchRecord> records;
-  std::vector<char> ran;
-  dlp_trace *trace = nullptr;
-  /* profiling: event-timed propagation launches */
-  int profiling = 0;
-  PropTiming timing;
-
-  template <class T>
-  int alloc(T **p, long long count) {
-    size_t bytes = (size_t)std::max<long long>(count, 1) * sizeof(T);
-    void *q = nullptr;
-    hipError_t e = hipMalloc(&q, bytes);
-    if (e != hipSuccess) {
-      (void)hipGetLastError();
-      dlp_set_errorf_("hipMalloc of %zu bytes failed (%s)", bytes, hipGetErrorString(e));
-      return DLP_ERR_NOMEM;
-    }
-    allocs.push_back(q);
-    deviceBytes += (int64_t)bytes;
-    *p = (T *)q;
-    return DLP_OK;
-  }
-  template <class T>
-  int upload(T **p, const T *host, long long count) {
-    int rc = alloc(p, count);
-    if (rc != DLP_OK) return rc;
-    if (count > 0) DLP_HIP(hipMemcpy(*p, host, (size_t)count * sizeof(T), hipMemcpyHostToDevice));
-    return DLP_OK;
-  }
-  void release() {
-    if (stream) (void)hipStreamSynchronize(stream);
-    for (void *p : allocs) (void)hipFree(p);
-    allocs.clear();
-    for (hipEvent_t e : timing.ev) (void)hipEventDestroy(e);
-    timing.ev.clear();
-    for (hipEvent_t e : evStart) (void)hipEventDestroy(e);
-    for (hipEvent_t e : evEnd) (void)hipEventDestroy(e);
-    evStart.clear();
-    evEnd.clear();
-    if (pinCtr) (void)hipHostFree(pinCtr);
-    pinCtr = nullptr;
-    if (ownStream && stream) (void)hipStreamDestroy(stream);
-    stream = nullptr;
-  }
-};
+#include "dlp_context.h"
 
 namespace {
 
@@ -230,7 +159,9 @@ int validate_inputs(const dlp_config *cfg, const dlp_graph_view *g, const float 
   return DLP_OK;
 }
 
-DBatch make_batch(const dlp_context *c, int b) {
+}  // namespace
+
+DBatch dlp_ctx_make_batch(const dlp_context *c, int b) {
   DBatch bt{};
   bt.b = b;
   auto span = [](const std::vector<int64_t> &off, int b2, long long &n) {
@@ -266,6 +197,8 @@ DBatch make_batch(const dlp_context *c, int b) {
   }
   return bt;
 }
+
+namespace {
 
 int run_propagation(dlp_context *c, const PropArgs &a, int maxIter) {
   PropLoopConfig cfg;
@@ -659,29 +592,52 @@ extern "C" int dlp_context_run_batch(dlp_context *c, int b) {
     return DLP_ERR_STATE;
   }
   hipStream_t s = c->stream;
-  const DState &d = c->d;
-  DBatch bt = make_batch(c, b);
-  int rc;
+  DBatch bt = dlp_ctx_make_batch(c, b);
+  int rc, markTag = 0;
   DLP_HIP(hipEventRecord(c->evStart[b], s));
+  if ((rc = dlp_ctx_batch_pre(c, b, bt, &markTag)) != DLP_OK) return rc;
+  if (c->trace && (rc = trace_pre_propagation(c, bt, markTag)) != DLP_OK) return rc;
+  if ((rc = run_propagation(c, dlp_ctx_prop_args(c), c->cfg.maxIterations)) != DLP_OK) return rc;
+  if ((rc = dlp_ctx_batch_post(c, b, bt)) != DLP_OK) return rc;
+  DLP_HIP(hipEventRecord(c->evEnd[b], s));
+  if (c->trace) {
+    DLP_HIP(hipStreamSynchronize(s));
+    DevBatchRecord r;
+    DLP_HIP(hipMemcpy(&r, c->d.records + b, sizeof(r), hipMemcpyDeviceToHost));
+    c->trace->numFrontierSizes = r.iterations;
+    int nc = std::min(std::min(r.iterations, c->trace->maxFrontierSizes), c->iterSizesCap);
+    if (c->trace->frontierSizes && nc > 0)
+      DLP_HIP(hipMemcpy(c->trace->frontierSizes, c->iterSizes, sizeof(int) * nc,
+                        hipMemcpyDeviceToHost));
+  }
+  return DLP_OK;
+}
+
+int dlp_ctx_batch_pre(dlp_context *c, int b, const DBatch &bt, int *markTag) {
+  hipStream_t s = c->stream;
+  const DState &d = c->d;
+  int rc;
   if ((rc = dlp_batch_begin(d, bt, s)) != DLP_OK) return rc;
-  const int markTag = ++c->markSeq;
-  if ((rc = dlp_csr_update(d, bt, markTag, s)) != DLP_OK) return rc;
+  *markTag = ++c->markSeq;
+  if ((rc = dlp_csr_update(d, bt, *markTag, s)) != DLP_OK) return rc;
   if (bt.nIns > 0 && bt.nE > 0) {
     if (c->lastCCBatch >= 0) {
-      DBatch prev = make_batch(c, c->lastCCBatch);
+      DBatch prev = dlp_ctx_make_batch(c, c->lastCCBatch);
       if ((rc = dlp_clear_component_ids(d, prev.ins, prev.nIns, s)) != DLP_OK) return rc;
     }
     if ((rc = dlp_sorted_inserted(d, s)) != DLP_OK) return rc;
-    if ((rc = dlp_sparsify(d, bt, c->tau, markTag, s)) != DLP_OK) return rc;
+    if ((rc = dlp_sparsify(d, bt, c->tau, *markTag, s)) != DLP_OK) return rc;
     if ((rc = dlp_components(d, bt, s)) != DLP_OK) return rc;
     if ((rc = dlp_label_init(d, bt, s)) != DLP_OK) return rc;
     c->lastCCBatch = b;
   } else if (bt.nIns > 0) {
     if ((rc = dlp_labels_no_edges(d, bt, s)) != DLP_OK) return rc;
   }
-  if ((rc = dlp_frontier0(d, c->N, s)) != DLP_OK) return rc;
-  if (c->trace && (rc = trace_pre_propagation(c, bt, markTag)) != DLP_OK) return rc;
+  return dlp_frontier0(d, c->N, s);
+}
 
+PropArgs dlp_ctx_prop_args(const dlp_context *c) {
+  const DState &d = c->d;
   PropArgs a{};
   a.rowBeg = d.fRow;
   a.rowEnd = nullptr;
@@ -700,21 +656,21 @@ extern "C" int dlp_context_run_batch(dlp_context *c, int b) {
   a.iterSizesCap = c->iterSizesCap;
   a.profile = c->profiling >= 2;
   a.delta = c->cfg.delta;
-  if ((rc = run_propagation(c, a, c->cfg.maxIterations)) != DLP_OK) return rc;
-  if ((rc = dlp_batch_end(d, bt, s)) != DLP_OK) return rc;
-  DLP_HIP(hipEventRecord(c->evEnd[b], s));
+  return a;
+}
+
+void dlp_ctx_advance_host(dlp_context *c, int b, const DBatch &bt) {
+  ++c->markSeq;
+  if (bt.nIns > 0 && bt.nE > 0) c->lastCCBatch = b;
   c->ran[b] = 1;
   c->nextBatch = b + 1;
-  if (c->trace) {
-    DLP_HIP(hipStreamSynchronize(s));
-    DevBatchRecord r;
-    DLP_HIP(hipMemcpy(&r, d.records + b, sizeof(r), hipMemcpyDeviceToHost));
-    c->trace->numFrontierSizes = r.iterations;
-    int nc = std::min(std::min(r.iterations, c->trace->maxFrontierSizes), c->iterSizesCap);
-    if (c->trace->frontierSizes && nc > 0)
-      DLP_HIP(hipMemcpy(c->trace->frontierSizes, c->iterSizes, sizeof(int) * nc,
-                        hipMemcpyDeviceToHost));
-  }
+}
+
+int dlp_ctx_batch_post(dlp_context *c, int b, const DBatch &bt) {
+  int rc = dlp_batch_end(c->d, bt, c->stream);
+  if (rc != DLP_OK) return rc;
+  c->ran[b] = 1;
+  c->nextBatch = b + 1;
   return DLP_OK;
 }
 
@@ -797,12 +753,19 @@ extern "C" int dlp_context_batch_stats(dlp_context *c, int b, dlp_batch_stats *o
     dlp_set_errorf_("batch %d has not run since the last reset", b);
     return DLP_ERR_STATE;
   }
+  int rc = dlp_ctx_fill_stats(c, b, out);
+  if (rc != DLP_OK) return rc;
+  float ms = 0.0f;
+  DLP_HIP(hipEventElapsedTime(&ms, c->evStart[b], c->evEnd[b]));
+  out->timeMs = ms;
+  return DLP_OK;
+}
+
+int dlp_ctx_fill_stats(dlp_context *c, int b, dlp_batch_stats *out) {
   DLP_HIP(hipStreamSynchronize(c->stream));
   DevBatchRecord r;
   DLP_HIP(hipMemcpy(&r, c->d.records + b, sizeof(r), hipMemcpyDeviceToHost));
-  float ms = 0.0f;
-  DLP_HIP(hipEventElapsedTime(&ms, c->evStart[b], c->evEnd[b]));
-  DBatch bt = make_batch(c, b);
+  DBatch bt = dlp_ctx_make_batch(c, b);
   out->batch = b;
   out->numInserted = bt.nIns;
   out->numDeleted = bt.nDel;
@@ -815,7 +778,7 @@ extern "C" int dlp_context_batch_stats(dlp_context *c, int b, dlp_batch_stats *o
   out->iterations = r.iterations;
   out->numTouched = r.touched;
   out->updates = (int64_t)r.updates;
-  out->timeMs = ms;
+  out->timeMs = 0.0f;
   out->accuracy = r.total > 0 ? (float)r.correct / (float)r.total : 1.0f;
   return DLP_OK;
 }
@@ -843,7 +806,7 @@ extern "C" int dlp_run_dynlp(const dlp_config *cfg, const dlp_graph_view *g,
     if (batchTimes) batchTimes[b] = st.timeMs;
     if (batchAccuracy) batchAccuracy[b] = st.accuracy;
     if (cfg->verbose) {
-      DBatch bt = make_batch(c, b);
+      DBatch bt = dlp_ctx_make_batch(c, b);
       std::printf("--- Batch %d ---\n  Inserting %d vertices (%d GT), deleting %d\n", b, bt.nIns,
                   bt.nGt, bt.nDel);
       if (bt.nIns > 0 && bt.nE > 0) {

@@ -405,30 +405,42 @@ constexpr int kRowHasOut = 0x40000000;
 /* LDS words past the working set (rounding of the aligned sections) */
 constexpr int kLdsSlack = 64;
 
+/* static LDS of the local passes */
+struct LocalShared {
+  int sh[SH_COUNT];
+  int misc[4];        /* frontier size broadcast of the work-group drivers */
+  int tmp[17];
+  int dummyStamp[64]; /* INT_MAX, set by the kernel before the first pass */
+  int dummyOut[64];
+};
+
+/* lane-local work counters of the local passes */
+struct LocalProf {
+  unsigned U = 0, E = 0, C = 0, A = 0;
+};
+
+/* Runs LDS-resident passes from iteration k (updated) until the frontier
+ * empties, k reaches maxIter, or the working set outgrows the LDS budget;
+ * returns the DLP_LOCAL_* reason.  All LT threads of the work-group call
+ * it; `pool` is the dynamic LDS (l.ldsWords words). */
 template <int LT, int SR>
-__global__ __launch_bounds__(LT) void k_propagate_local(PropArgs a, LocalArgs l, int k0,
-                                                        int maxIter) {
+__device__ __forceinline__ int local_passes(const PropArgs &a, const LocalArgs &l, int &k, int maxIter,
+                            int *pool, LocalShared &S, LocalProf &prof) {
   static_assert(SR <= kLaneRowMax, "padded stride above the lane row limit");
   /* padded rows are SR + 1 records apart: with 2*(SR+1) words (SR a power
    * of two) the rows of neighbouring core vertices start in different LDS
    * banks, so a wave's lanes reading position j of their rows do not all
    * collide in a few banks */
   constexpr int RS = SR + 1;
-  extern __shared__ int4 pool4[];
-  int *pool = (int *)pool4;
-  __shared__ int sh[SH_COUNT];
-  __shared__ int tmp[17];
-  __shared__ int dummyStamp[64], dummyOut[64];
+  int *sh = S.sh, *tmp = S.tmp, *dummyStamp = S.dummyStamp, *dummyOut = S.dummyOut;
   DevCounters *c = a.ctr;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int P = l.ldsWords;
   const int stampBase = c->stampBase;
-  int k = k0;
   int reason = 0;
   bool useCore = true;
-  unsigned profU = 0, profE = 0, profC = 0, profA = 0;
+  unsigned &profU = prof.U, &profE = prof.E, &profC = prof.C, &profA = prof.A;
   unsigned long long passUpdates = 0; /* tid 0: frontier sizes of this pass */
-  if (tid < 64) dummyStamp[tid] = DLP_INT_MAX;
 
   while (true) {  // one pass = build the LDS working set, iterate, write back
     /* every thread has read the previous pass's sh[] values */
@@ -738,24 +750,285 @@ __global__ __launch_bounds__(LT) void k_propagate_local(PropArgs a, LocalArgs l,
     }
     useCore = true;  // rebuild with the grown core at iteration k
   }
-  if (tid == 0) {
-    c->nextIter = k;
-    c->localReason = reason;
-  }
-  if (a.profile) {
+  return reason;
+}
+
+/* adds the work counters of the local passes to the graph's totals */
+__device__ __forceinline__ void local_prof_flush(const PropArgs &a, LocalProf &prof) {
+  if (!a.profile) return;
+  DevCounters *c = a.ctr;
 #pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) {
-      profE += __shfl_xor(profE, off, 64);
-      profC += __shfl_xor(profC, off, 64);
-      profA += __shfl_xor(profA, off, 64);
-    }
-    if (lane == 0) {
-      if (profE) atomicAdd(&c->profLocalEdges, (unsigned long long)profE);
-      if (profC) atomicAdd(&c->profLocalChangedEdges, (unsigned long long)profC);
-      if (profA) atomicAdd(&c->profLocalAppended, (unsigned long long)profA);
-    }
-    if (tid == 0 && profU) atomicAdd(&c->profLocalUpdates, (unsigned long long)profU);
+  for (int off = 32; off >= 1; off >>= 1) {
+    prof.E += __shfl_xor(prof.E, off, 64);
+    prof.C += __shfl_xor(prof.C, off, 64);
+    prof.A += __shfl_xor(prof.A, off, 64);
   }
+  if ((threadIdx.x & 63) == 0) {
+    if (prof.E) atomicAdd(&c->profLocalEdges, (unsigned long long)prof.E);
+    if (prof.C) atomicAdd(&c->profLocalChangedEdges, (unsigned long long)prof.C);
+    if (prof.A) atomicAdd(&c->profLocalAppended, (unsigned long long)prof.A);
+  }
+  if (threadIdx.x == 0 && prof.U) atomicAdd(&c->profLocalUpdates, (unsigned long long)prof.U);
+  prof = LocalProf{};
+}
+
+template <int LT, int SR>
+__global__ __launch_bounds__(LT) void k_propagate_local(PropArgs a, LocalArgs l, int k0,
+                                                        int maxIter) {
+  extern __shared__ int4 pool4[];
+  __shared__ LocalShared S;
+  if (threadIdx.x < 64) S.dummyStamp[threadIdx.x] = DLP_INT_MAX;  // first pass starts with a barrier
+  LocalProf prof;
+  int k = k0;
+  const int reason = local_passes<LT, SR>(a, l, k, maxIter, (int *)pool4, S, prof);
+  if (threadIdx.x == 0) {
+    a.ctr->nextIter = k;
+    a.ctr->localReason = reason;
+  }
+  local_prof_flush(a, prof);
+}
+
+/* ------------------------------------------------------------------------ */
+/* one work-group iterating over HBM (graphs whose working set does not fit */
+/* the LDS), and the multi-graph driver                                     */
+/* ------------------------------------------------------------------------ */
+
+/* Frontier vertices with rows of at most L entries, one per lane, reading
+ * the row, the labels and the stamps in HBM; appends with one atomic per
+ * wave.  All 64 lanes call it. */
+template <int L>
+__device__ __forceinline__ void wg_lane_rows(const PropArgs &a, const float *Fold, float *Fnew,
+                                             bool act, int u, int beg, int len, int stampNext,
+                                             int stampBase, int *out, int *outCount,
+                                             unsigned &profE, unsigned &profC,
+                                             unsigned &profA) {
+  int vv[L];
+  float ww[L], ff[L];
+#pragma unroll
+  for (int j = 0; j < L; ++j) {
+    int v = -1;
+    float w = 0.0f;
+    if (j < len) {
+      v = a.colInd[beg + j];
+      w = a.values[beg + j];
+    }
+    if (v >= 0 && a.status && a.status[v] == DLP_VERTEX_DELETED) v = -1;
+    vv[j] = v;
+    ww[j] = v >= 0 ? w : 0.0f;
+  }
+#pragma unroll
+  for (int j = 0; j < L; ++j) ff[j] = vv[j] >= 0 ? Fold[vv[j]] : 0.0f;
+  const float oldLabel = Fold[u];
+  float W, S;
+  dlp::lane_row_sums<L>(
+      L,
+      [&](int j, float &w, float &s) {
+        w = ww[j];
+        s = ww[j] * ff[j];
+      },
+      W, S);
+  const float nl = dlp::new_label(W, S, oldLabel);
+  const bool changed = act && fabsf(nl - oldLabel) > a.delta;
+  if (act) {
+    Fnew[u] = nl;
+    profE += (unsigned)len;
+    if (changed) profC += (unsigned)len + 1u;
+  }
+  int olds[L + 1];
+#pragma unroll
+  for (int j = 0; j < L; ++j) {
+    int o = DLP_INT_MAX;
+    if (changed && vv[j] >= 0) o = atomicMax(&a.iterStamp[vv[j]], stampNext);
+    olds[j] = o;
+  }
+  {
+    int o = DLP_INT_MAX;
+    if (changed) o = atomicMax(&a.iterStamp[u], stampNext);
+    olds[L] = o;
+  }
+  unsigned hit = 0u, first = 0u;
+#pragma unroll
+  for (int j = 0; j <= L; ++j) {
+    hit |= (olds[j] < stampNext ? 1u : 0u) << j;
+    first |= (olds[j] < stampBase ? 1u : 0u) << j;
+  }
+  const int lane = dlp_lane();
+  const int cnt = __popc(hit);
+  profA += (unsigned)cnt;
+  {
+    const int incl = dlp_wave_inclusive_sum(cnt);
+    const int total = __builtin_amdgcn_readlane(incl, 63);
+    if (total > 0) {  // wave-uniform
+      int wbase = 0;
+      if (lane == 63) wbase = atomicAdd(outCount, total);
+      wbase = __builtin_amdgcn_readlane(wbase, 63);
+      int pos = wbase + incl - cnt;
+#pragma unroll
+      for (int j = 0; j <= L; ++j)
+        if ((hit >> j) & 1u) out[pos++] = j < L ? vv[j < L ? j : 0] : u;
+    }
+  }
+  if (a.touched) {
+    const int tc = __popc(first);
+    const int incl = dlp_wave_inclusive_sum(tc);
+    const int total = __builtin_amdgcn_readlane(incl, 63);
+    if (total > 0) {  // wave-uniform
+      int wbase = 0;
+      if (lane == 63) wbase = atomicAdd(&a.ctr->touched, total);
+      wbase = __builtin_amdgcn_readlane(wbase, 63);
+      int pos = wbase + incl - tc;
+#pragma unroll
+      for (int j = 0; j <= L; ++j)
+        if ((first >> j) & 1u) a.touched[pos++] = j < L ? vv[j < L ? j : 0] : u;
+    }
+  }
+}
+
+/* Iterations k, k+1, ... by one work-group over HBM, while the frontier is
+ * non-empty, k < maxIter, and not (frontier <= threshold and k >= resumeAt):
+ * the same global state transitions as one k_propagate launch each.  On
+ * return S.misc[0] holds the frontier size of iteration k. */
+template <int LT>
+__device__ __forceinline__ void wg_iterations(const PropArgs &a, int &k, int maxIter, int threshold,
+                              int resumeAt, LocalShared &S, LocalProf &prof) {
+  constexpr int kWgLaneRowMax = 16;
+  DevCounters *c = a.ctr;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int stampBase = c->stampBase;
+  while (true) {
+    __syncthreads();  // every thread has read the previous S.misc[0]
+    if (tid == 0) S.misc[0] = ld_agent(&c->qcount[k % 3]);
+    __syncthreads();
+    const int n = S.misc[0];
+    if (n == 0 || k >= maxIter || (n <= threshold && k >= resumeAt)) break;  // uniform
+    if (tid == 0) {
+      c->qcount[(k + 2) % 3] = 0;
+      if (k == 0) c->frontier0 = n;
+      if (a.iterSizes && k < a.iterSizesCap) a.iterSizes[k] = n;
+      c->iterUsed = k + 1;
+      c->updates += (unsigned long long)n;
+      prof.U += (unsigned)n;
+    }
+    const int stampNext = stampBase + k + 1;
+    const int *in = (k & 1) ? a.q1 : a.q0;
+    int *out = (k & 1) ? a.q0 : a.q1;
+    int *outCount = &c->qcount[(k + 1) % 3];
+    const float *Fold = (k & 1) ? a.lab1 : a.lab0;
+    float *Fnew = (k & 1) ? a.lab0 : a.lab1;
+    for (int base = 0; base < n; base += LT) {  // block-uniform
+      if (base + (tid & ~63) >= n) continue;     // wave-uniform: no vertex for this wave
+      const int item = base + tid;
+      const bool act = item < n;
+      const int u = act ? in[item] : 0;
+      const int beg = act ? a.rowBeg[u] : 0;
+      const int len = act ? (a.rowLen ? a.rowLen[u] : a.rowEnd[u] - beg) : 0;
+      const bool laneRow = act && len <= kWgLaneRowMax;
+      const int lenS = laneRow ? len : 0;
+      if (!__any(lenS > 4))
+        wg_lane_rows<4>(a, Fold, Fnew, laneRow, u, beg, lenS, stampNext, stampBase, out,
+                        outCount, prof.E, prof.C, prof.A);
+      else if (!__any(lenS > 8))
+        wg_lane_rows<8>(a, Fold, Fnew, laneRow, u, beg, lenS, stampNext, stampBase, out,
+                        outCount, prof.E, prof.C, prof.A);
+      else
+        wg_lane_rows<kWgLaneRowMax>(a, Fold, Fnew, laneRow, u, beg, lenS, stampNext,
+                                    stampBase, out, outCount, prof.E, prof.C, prof.A);
+      /* longer rows: the wave takes them one by one */
+      unsigned long long longMask = dlp_ballot(act && !laneRow);
+      while (longMask) {  // wave-uniform
+        const int src = __ffsll((long long)longMask) - 1;
+        longMask &= longMask - 1;
+        const int lu = __shfl(u, src, 64), lbeg = __shfl(beg, src, 64),
+                  llen = __shfl(len, src, 64);
+        float W, Sm;
+        dlp::group_row_sums<64>(lane, llen, GlobalPos{a.colInd, a.values, Fold, a.status, lbeg},
+                                W, Sm);
+        int changed = 0;
+        if (lane == 0) {
+          const float oldLabel = Fold[lu];
+          const float nl = dlp::new_label(W, Sm, oldLabel);
+          Fnew[lu] = nl;
+          changed = fabsf(nl - oldLabel) > a.delta;
+          prof.E += (unsigned)llen;
+          if (changed) prof.C += (unsigned)llen + 1u;
+        }
+        changed = __shfl(changed, 0, 64);
+        if (!changed) continue;  // wave-uniform
+        for (int p0 = 0; p0 <= llen; p0 += 64) {  // wave-uniform
+          const int p = p0 + lane;
+          int v = -1;
+          bool pred = false;
+          if (p < llen) {
+            v = a.colInd[lbeg + p];
+            pred = v >= 0 && !(a.status && a.status[v] == DLP_VERTEX_DELETED);
+          } else if (p == llen) {
+            v = lu;
+            pred = true;
+          }
+          const int old = pred ? atomicMax(&a.iterStamp[v], stampNext) : DLP_INT_MAX;
+          const bool app = pred && old < stampNext;
+          prof.A += app ? 1u : 0u;
+          const int slot = dlp_wave_append(outCount, app);
+          if (slot >= 0) out[slot] = v;
+          if (a.touched) {
+            const int ts = dlp_wave_append(&c->touched, app && old < stampBase);
+            if (ts >= 0) a.touched[ts] = v;
+          }
+        }
+      }
+    }
+    wg_global_sync();  // this iteration's labels, stamps and queue for every wave
+    ++k;
+  }
+}
+
+/* the HBM-iteration work counters of one work-group */
+__device__ __forceinline__ void wg_prof_flush(const PropArgs &a, LocalProf &prof) {
+  if (!a.profile) return;
+  DevCounters *c = a.ctr;
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    prof.E += __shfl_xor(prof.E, off, 64);
+    prof.C += __shfl_xor(prof.C, off, 64);
+    prof.A += __shfl_xor(prof.A, off, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    if (prof.E) atomicAdd(&c->profEdges, (unsigned long long)prof.E);
+    if (prof.C) atomicAdd(&c->profChangedEdges, (unsigned long long)prof.C);
+    if (prof.A) atomicAdd(&c->profAppended, (unsigned long long)prof.A);
+  }
+  prof = LocalProf{};
+}
+
+/* One work-group per graph runs the whole propagation of its graph's batch:
+ * LDS-resident passes while the frontier is at most the threshold (and no
+ * overflow in the last `backoff` iterations), HBM iterations otherwise. */
+template <int LT, int SR>
+__global__ __launch_bounds__(LT) void k_propagate_multi(const GraphProp *graphs) {
+  extern __shared__ int4 pool4[];
+  __shared__ LocalShared S;
+  const GraphProp &gp = graphs[blockIdx.x];
+  const PropArgs a = gp.a;
+  const LocalArgs l = gp.l;
+  const int maxIter = gp.maxIter, threshold = gp.threshold, backoff = gp.backoff;
+  if (threadIdx.x < 64) S.dummyStamp[threadIdx.x] = DLP_INT_MAX;
+  LocalProf lprof, gprof;
+  int k = 0, resumeAt = 0;
+  if (l.ldsWords > 0 && threshold > 0) {
+    while (true) {
+      wg_iterations<LT>(a, k, maxIter, threshold, resumeAt, S, gprof);
+      const int n = S.misc[0];
+      if (n == 0 || k >= maxIter) break;  // uniform
+      const int reason = local_passes<LT, SR>(a, l, k, maxIter, (int *)pool4, S, lprof);
+      if (reason != DLP_LOCAL_TOO_BIG) break;
+      resumeAt = k + backoff;
+      wg_global_sync();
+    }
+  } else {
+    wg_iterations<LT>(a, k, maxIter, -1, maxIter, S, gprof);
+  }
+  local_prof_flush(a, lprof);
+  wg_prof_flush(a, gprof);
 }
 
 }  // namespace
@@ -792,6 +1065,8 @@ int dlp_local_lds_bytes(int *bytes) {
   v &= ~255;
 #define DLP_SET_LDS(I, T, R)                                                       \
   DLP_HIP(hipFuncSetAttribute((const void *)k_propagate_local<T, R>,              \
+                              hipFuncAttributeMaxDynamicSharedMemorySize, v));     \
+  DLP_HIP(hipFuncSetAttribute((const void *)k_propagate_multi<T, R>,              \
                               hipFuncAttributeMaxDynamicSharedMemorySize, v));
   DLP_LOCAL_SHAPES(DLP_SET_LDS)
 #undef DLP_SET_LDS
@@ -815,5 +1090,24 @@ int dlp_launch_propagate_local(const PropArgs &a, const LocalArgs &l, int k, int
       return DLP_ERR_INVALID;
   }
   DLP_LAUNCHED("k_propagate_local");
+  return DLP_OK;
+}
+
+int dlp_launch_propagate_multi(const GraphProp *graphs, int numGraphs, int shape, int ldsWords,
+                               hipStream_t s) {
+  if (numGraphs <= 0) return DLP_OK;
+  const size_t lds = (size_t)ldsWords * 4;
+  switch (shape) {
+#define DLP_LAUNCH_MULTI(I, T, R)                                                        \
+  case I:                                                                               \
+    hipLaunchKernelGGL((k_propagate_multi<T, R>), dim3(numGraphs), dim3(T), lds, s, graphs); \
+    break;
+    DLP_LOCAL_SHAPES(DLP_LAUNCH_MULTI)
+#undef DLP_LAUNCH_MULTI
+    default:
+      dlp_set_errorf_("propagation shape %d not in [0, %d)", shape, kNumLocalShapes);
+      return DLP_ERR_INVALID;
+  }
+  DLP_LAUNCHED("k_propagate_multi");
   return DLP_OK;
 }

@@ -308,3 +308,94 @@ class Context:
         s = _lib.BatchStats()
         check(self._lib.dlp_context_batch_stats(self._h, b, C.byref(s)))
         return {name: getattr(s, name) for name, _ in s._fields_}
+
+
+class Multi:
+    """Many independent graphs, one propagation launch per batch index
+    (dlp_multi_*).  `graphs` is a list of Inputs."""
+
+    def __init__(self, graphs, cfgs=None):
+        self._lib = load()
+        self.graphs = list(graphs)
+        n = len(self.graphs)
+        self.cfgs = cfgs if cfgs is not None else [make_config(g) for g in self.graphs]
+        self._cfgArr = (_lib.Config * n)(*self.cfgs)
+        self._gv = (_lib.GraphView * n)(*[g.graph_view() for g in self.graphs])
+        self._bv = (_lib.BatchView * n)(*[g.batch_view() for g in self.graphs])
+        self._tl = (C.POINTER(C.c_float) * n)(*[_ptr(g.trueLabels, C.c_float)
+                                                 for g in self.graphs])
+        self._h = C.c_void_p()
+        check(self._lib.dlp_multi_create(C.byref(self._h), n, self._cfgArr, self._gv, self._tl,
+                                         self._bv))
+
+    def close(self):
+        if self._h:
+            self._lib.dlp_multi_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_tuning(self, localShape=0, thresholdPercent=0, backoff=0, useGraphs=0):
+        """0 keeps a setting; useGraphs 1 = HIP-graph replay, 2 = enqueue."""
+        t = _lib.MultiTuning(localShape, thresholdPercent, backoff, useGraphs)
+        check(self._lib.dlp_multi_set_tuning(self._h, C.byref(t)))
+
+    def tuning(self) -> dict:
+        t = _lib.MultiTuning()
+        check(self._lib.dlp_multi_get_tuning(self._h, C.byref(t)))
+        return {name: getattr(t, name) for name, _ in t._fields_}
+
+    def reset(self):
+        check(self._lib.dlp_multi_reset(self._h))
+
+    def run_batch(self, b: int):
+        check(self._lib.dlp_multi_run_batch(self._h, b))
+
+    def run_all(self):
+        check(self._lib.dlp_multi_run_all(self._h))
+
+    def synchronize(self):
+        check(self._lib.dlp_multi_synchronize(self._h))
+
+    def labels(self, g: int) -> np.ndarray:
+        out = np.empty(self.graphs[g].numNodes, np.float32)
+        check(self._lib.dlp_multi_read_labels(self._h, g, _ptr(out, C.c_float)))
+        return out
+
+    def stats(self, g: int, b: int) -> dict:
+        st = _lib.BatchStats()
+        check(self._lib.dlp_multi_batch_stats(self._h, g, b, C.byref(st)))
+        return {name: getattr(st, name) for name, _ in st._fields_}
+
+    def batch_time_ms(self, b: int) -> float:
+        ms = C.c_float()
+        check(self._lib.dlp_multi_batch_time(self._h, b, C.byref(ms)))
+        return ms.value
+
+    def device_bytes(self) -> int:
+        v = C.c_int64()
+        check(self._lib.dlp_multi_device_bytes(self._h, C.byref(v)))
+        return v.value
+
+
+def batched_workload(numGraphs: int = 256, numBatches: int = 10, seed: int = 42,
+                     minNodes: int = 2000, maxNodes: int = 200000, degrees=(3, 5, 7)):
+    """The batched workload: numGraphs graphs with vertex counts
+    log-uniform in [minNodes, maxNodes] and average degree drawn from
+    `degrees`, both from numpy.random.default_rng(seed); graph g is generated
+    with the reference generator and seed + g."""
+    rng = np.random.default_rng(seed)
+    sizes = np.rint(np.exp(rng.uniform(np.log(minNodes), np.log(maxNodes), numGraphs)))
+    degs = rng.choice(np.asarray(degrees), numGraphs)
+    return [generate(int(n), int(d), numBatches, seed + g)
+            for g, (n, d) in enumerate(zip(sizes, degs))]
