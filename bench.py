@@ -221,6 +221,92 @@ def batched_main(args) -> int:
     return 0
 
 
+def serving_main(args) -> int:
+    """Serving form: tenant graphs resident in HBM, update batches arriving
+    as a Poisson process, admitted per step by slots and HBM budget and run
+    together, against the same arrivals served one batch at a time with the
+    plain single-graph path."""
+    import labelpropcuda_amd as L
+    t0 = time.perf_counter()
+
+    def progress(msg):
+        print(f"[bench serving {time.perf_counter() - t0:7.1f} s] {msg}", file=sys.stderr,
+              flush=True)
+
+    rng = np.random.default_rng(1042)
+    T, R = args.tenants, args.requests
+    sizesN = np.rint(np.exp(rng.uniform(np.log(1e4), np.log(args.max_tenant_nodes), T)))
+    reqTenant = rng.integers(0, T, R)
+    reqSize = np.rint(np.exp(rng.uniform(np.log(100), np.log(1e5), R))).astype(int)
+    tenants = []
+    for t in range(T):
+        sizes = [0] + [int(k) for k in reqSize[reqTenant == t]]
+        tenants.append(L.generate_stream(int(sizesN[t]), 5, sizes, 1042 + t))
+        if t % 8 == 7:
+            progress(f"generated {t + 1} tenants")
+    progress(f"generated {T} tenants, {int(sizesN.sum())} vertices, {R} requests")
+    m = L.Multi(tenants)
+    progress("tenants resident")
+
+    def run(arrivals, plain):
+        m.reset()
+        m.run_next(list(range(T)))  # seed batches
+        m.synchronize()
+        reqs = list(zip(reqTenant.tolist(), arrivals.tolist()))
+        done, steps = m.serve(reqs, maxSlots=args.slots, hbmBudgetBytes=args.hbm_budget,
+                              plain=plain)
+        m.synchronize()
+        lat = 1000.0 * (done - arrivals)
+        upd = sum(m.stats(t, b)["updates"] for t in range(T)
+                  for b in range(1, tenants[t].numBatches))
+        return {"makespan_s": float(done.max()), "steps": steps, "updates": int(upd),
+                "updates_per_s": upd / float(done.max()),
+                "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99))}
+
+    zero = np.zeros(R)
+    sat_plain = run(zero, True)
+    progress(f"plain, saturated: {sat_plain['makespan_s']:.1f} s")
+    sat_batched = run(zero, False)
+    progress(f"batched, saturated: {sat_batched['makespan_s']:.1f} s")
+    assert sat_plain["updates"] == sat_batched["updates"]
+    service = sat_plain["makespan_s"] / R  # mean plain service time
+    sweep = []
+    for f in args.rates:
+        gaps = rng.exponential(service / f, R)
+        arrivals = np.cumsum(gaps) - gaps[0]
+        row = {"rate_x_plain_capacity": f, "arrivals_per_s": f / service,
+               "batched": run(arrivals, False), "plain": run(arrivals, True)}
+        sweep.append(row)
+        progress(f"rate {f} x capacity: batched p99 {row['batched']['p99_ms']:.1f} ms, "
+                 f"plain p99 {row['plain']['p99_ms']:.1f} ms")
+    m.close()
+    value = sat_batched["updates_per_s"]
+    out = {
+        "metric": f"affected-vertex updates/s (serving, {T} tenants, saturated)",
+        "value": value,
+        "unit": "updates/s",
+        "n_gpus": 1,
+        "steps": sat_batched["steps"],
+        "warmup": 0,
+        "ms_per_step": 1000.0 * sat_batched["makespan_s"] / max(sat_batched["steps"], 1),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (reference graph generator, tenant t seed 1042+t, streamed batches)",
+        "config": {"workload": f"{T} tenants, vertices log-uniform in [10000, "
+                               f"{int(args.max_tenant_nodes)}], degree 5; {R} batches of "
+                               "log-uniform 100-100000 vertices (90/1/9)",
+                   "slots": args.slots, "hbm_budget_bytes": args.hbm_budget,
+                   "updates": sat_batched["updates"]},
+        "saturated": {"batched": sat_batched, "plain": sat_plain,
+                      "speedup_vs_plain": value / sat_plain["updates_per_s"]},
+        "rate_sweep": sweep,
+    }
+    print(json.dumps(out))
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -237,9 +323,19 @@ def main() -> int:
     ap.add_argument("--local-shape", type=int, default=0,
                     help="work-group shape of the LDS-resident kernel (1-based)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--workload", choices=("c2", "batched"), default="c2",
+    ap.add_argument("--workload", choices=("c2", "batched", "serving"), default="c2",
                     help="c2: one 500,000-vertex graph (default); batched: many graphs, "
-                         "one propagation launch per batch index")
+                         "one propagation launch per batch index; serving: arrival-driven "
+                         "continuous batching over resident tenant graphs")
+    ap.add_argument("--tenants", type=int, default=64, help="tenant graphs (serving)")
+    ap.add_argument("--requests", type=int, default=128, help="update batches (serving)")
+    ap.add_argument("--max-tenant-nodes", type=float, default=5e6,
+                    help="largest tenant graph (serving)")
+    ap.add_argument("--slots", type=int, default=64, help="batches per step (serving)")
+    ap.add_argument("--hbm-budget", type=int, default=8 << 30,
+                    help="HBM working-set budget of one step in bytes (serving)")
+    ap.add_argument("--rates", type=float, nargs="*", default=[0.5, 1.0, 2.0, 4.0],
+                    help="arrival rates as multiples of the plain path's capacity (serving)")
     ap.add_argument("--graphs", type=int, default=256, help="graphs of the batched workload")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="after the run, copy this many MiB device to device with the "
@@ -256,6 +352,8 @@ def main() -> int:
         return 1
     if args.workload == "batched":
         return batched_main(args)
+    if args.workload == "serving":
+        return serving_main(args)
 
     inputs = L.generate(args.nodes, args.degree, args.batches, 42)
     cfg = L.make_config(inputs)

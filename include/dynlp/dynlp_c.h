@@ -317,6 +317,15 @@ typedef struct dlp_inputs dlp_inputs;
 int dlp_inputs_generate(dlp_inputs **out, int totalNodes, int avgDegree,
                         int numBatches, unsigned int seed,
                         float groundTruthFraction, float deletedFraction);
+/* Arrival-driven variant (no counterpart in the reference): batch 0 is
+ * the GT seed batch of dlp_inputs_generate; batch b >= 1 inserts the next
+ * batchSizes[b] vertices of one shuffled order with the same GT fraction,
+ * deletions drawn uniformly from the active non-GT vertices, and edges in
+ * batch max(b_u, b_v). */
+int dlp_inputs_generate_stream(dlp_inputs **out, int totalNodes, int avgDegree,
+                               const int *batchSizes, int numBatches,
+                               unsigned int seed, float groundTruthFraction,
+                               float deletedFraction);
 /* Views into the handle's memory (valid until dlp_inputs_free). */
 int dlp_inputs_graph(const dlp_inputs *in, dlp_graph_view *graph,
                      const float **trueLabels);
@@ -361,6 +370,30 @@ int dlp_multi_read_labels(dlp_multi *m, int graph, float *labels);
 int dlp_multi_batch_stats(dlp_multi *m, int graph, int b, dlp_batch_stats *out);
 int dlp_multi_batch_time(dlp_multi *m, int b, float *ms);
 int dlp_multi_device_bytes(const dlp_multi *m, int64_t *bytes);
+
+/* Serving.  Runs the next batch of each listed graph (distinct indices)
+ * together: their pipelines, one propagation launch, their batch ends. */
+int dlp_multi_run_next(dlp_multi *m, const int *graphs, int count);
+
+typedef struct dlp_serve_request {
+  int graph;       /* the request is this graph's next batch          */
+  double arrival;  /* seconds after the start of dlp_multi_serve      */
+} dlp_serve_request;
+
+typedef struct dlp_serve_config {
+  int maxSlots;           /* batches admitted into one step (>= 1)    */
+  int64_t hbmBudgetBytes; /* HBM working set of the admitted batches
+                             (lists, G', sort scratch); 0 = unlimited  */
+  int plain;              /* 1: one batch per step, single-graph path  */
+} dlp_serve_config;
+
+/* Arrival-driven continuous batching: requests (sorted by arrival) are
+ * released at their arrival times; each step admits, in arrival order,
+ * at most one batch per graph, up to maxSlots batches and the HBM budget,
+ * and runs them together; completion[i] receives the time (seconds from
+ * the start) at which request i's labels were ready. */
+int dlp_multi_serve(dlp_multi *m, const dlp_serve_request *reqs, int numReqs,
+                    const dlp_serve_config *cfg, double *completion, int *steps);
 
 /* ====================================================================== */
 /* 5. Device-memory plumbing for callers without a GPU runtime of their   */

@@ -6,7 +6,7 @@ C-ABI used by the tests and the benchmark.
 """
 from ._lib import DynLPError, device_count, load  # noqa: F401
 from .dynlp import (Context, Inputs, Multi, TraceBuffers, batched_workload,  # noqa: F401
-                    generate, make_config, run_dynlp)
+                    generate, generate_stream, make_config, run_dynlp)
 
 __all__ = ["Context", "Inputs", "Multi", "TraceBuffers", "batched_workload", "generate",
-           "make_config", "run_dynlp", "DynLPError", "device_count", "load"]
+           "generate_stream", "make_config", "run_dynlp", "DynLPError", "device_count", "load"]

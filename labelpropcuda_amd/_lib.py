@@ -131,6 +131,14 @@ class MultiTuning(C.Structure):
     ]
 
 
+class ServeRequest(C.Structure):
+    _fields_ = [("graph", C.c_int), ("arrival", C.c_double)]
+
+
+class ServeConfig(C.Structure):
+    _fields_ = [("maxSlots", C.c_int), ("hbmBudgetBytes", C.c_int64), ("plain", C.c_int)]
+
+
 class PropProfile(C.Structure):
     _fields_ = [
         ("launches", C.c_int64),
@@ -208,6 +216,12 @@ SIGNATURES = {
     "dlp_multi_batch_stats": (C.c_int, [_VP, C.c_int, C.c_int, C.POINTER(BatchStats)]),
     "dlp_multi_batch_time": (C.c_int, [_VP, C.c_int, C.POINTER(C.c_float)]),
     "dlp_multi_device_bytes": (C.c_int, [_VP, C.POINTER(C.c_int64)]),
+    "dlp_multi_run_next": (C.c_int, [_VP, C.POINTER(C.c_int), C.c_int]),
+    "dlp_multi_serve": (C.c_int, [_VP, C.POINTER(ServeRequest), C.c_int, C.POINTER(ServeConfig),
+                                  C.POINTER(C.c_double), C.POINTER(C.c_int)]),
+    "dlp_inputs_generate_stream": (C.c_int, [C.POINTER(_VP), C.c_int, C.c_int,
+                                             C.POINTER(C.c_int), C.c_int, C.c_uint, C.c_float,
+                                             C.c_float]),
     "dlp_copy_device": (C.c_int, [_VP, _VP, C.c_size_t]),
     "dlp_device_malloc": (C.c_int, [C.POINTER(_VP), C.c_size_t]),
     "dlp_device_free": (C.c_int, [_VP]),

@@ -13,6 +13,8 @@
  * iterativePropagation.cu:274-327).
  */
 #include <algorithm>
+#include <chrono>
+#include <thread>
 #include <vector>
 
 #include "dlp_context.h"
@@ -33,6 +35,10 @@ struct dlp_multi {
   std::vector<hipEvent_t> evStart, evEnd;
   std::vector<char> ran;
   int nextBatch = 0;
+  /* run_next: descriptors of the graphs of one step */
+  std::vector<GraphProp> props;     /* per graph, batch-independent      */
+  GraphProp *dStep = nullptr;       /* device, G entries                 */
+  GraphProp *hStep = nullptr;       /* pinned staging, G entries         */
 
   void drop_graphs() {
     for (hipGraphExec_t &e : exec)
@@ -49,6 +55,10 @@ struct dlp_multi {
     for (GraphProp *p : dProps)
       if (p) (void)hipFree(p);
     dProps.clear();
+    if (dStep) (void)hipFree(dStep);
+    dStep = nullptr;
+    if (hStep) (void)hipHostFree(hStep);
+    hStep = nullptr;
     for (dlp_context *c : ctx) dlp_context_destroy(c);
     ctx.clear();
     for (hipEvent_t e : evStart) (void)hipEventDestroy(e);
@@ -63,21 +73,16 @@ struct dlp_multi {
 namespace {
 
 /* per-graph descriptors of batch b's propagation launch */
+GraphProp graph_prop(const dlp_multi *m, const dlp_context *c);
+
 int upload_props(dlp_multi *m) {
+  m->props.clear();
+  for (dlp_context *c : m->ctx) m->props.push_back(graph_prop(m, c));
   for (int b = 0; b < m->maxB; ++b) {
     std::vector<GraphProp> h;
-    for (dlp_context *c : m->ctx) {
-      if (b >= c->B) continue;
-      GraphProp g{};
-      g.a = dlp_ctx_prop_args(c);
-      g.l = c->local;
-      g.l.ldsWords = m->ldsWords;
-      g.maxIter = c->cfg.maxIterations;
-      const long long th = (long long)c->tuning.localThreshold * m->thresholdPct / 100;
-      g.threshold = m->thresholdPct < 0 ? 0 : (int)std::max<long long>(1, std::min<long long>(th, 1 << 30));
-      if (c->N >= (1 << 30)) g.threshold = 0;  // outside-vertex encoding limit
-      g.backoff = m->backoff;
-      h.push_back(g);
+    for (int gi = 0; gi < m->G; ++gi) {
+      if (b >= m->ctx[gi]->B) continue;
+      h.push_back(m->props[gi]);
     }
     m->counts[b] = (int)h.size();
     if (!m->dProps[b]) {
@@ -90,6 +95,21 @@ int upload_props(dlp_multi *m) {
                         hipMemcpyHostToDevice));
   }
   return DLP_OK;
+}
+
+/* the propagation descriptor of one graph (the same for every batch) */
+GraphProp graph_prop(const dlp_multi *m, const dlp_context *c) {
+  GraphProp g{};
+  g.a = dlp_ctx_prop_args(c);
+  g.l = c->local;
+  g.l.ldsWords = m->ldsWords;
+  g.maxIter = c->cfg.maxIterations;
+  const long long th = (long long)c->tuning.localThreshold * m->thresholdPct / 100;
+  g.threshold =
+      m->thresholdPct < 0 ? 0 : (int)std::max<long long>(1, std::min<long long>(th, 1 << 30));
+  if (c->N >= (1 << 30)) g.threshold = 0;  // outside-vertex encoding limit
+  g.backoff = m->backoff;
+  return g;
 }
 
 /* enqueue batch b of every graph on the shared stream */
@@ -215,6 +235,19 @@ extern "C" int dlp_multi_create(dlp_multi **out, int numGraphs, const dlp_config
     }
   }
   MTRY(upload_props(m));
+  {
+    void *p = nullptr;
+    if (hipMalloc(&p, sizeof(GraphProp) * m->G) != hipSuccess ||
+        hipHostMalloc((void **)&m->hStep, sizeof(GraphProp) * m->G, hipHostMallocDefault) !=
+            hipSuccess) {
+      if (p) (void)hipFree(p);
+      m->release();
+      delete m;
+      dlp_set_error_("dlp_multi_create: allocation of the step descriptors failed");
+      return DLP_ERR_NOMEM;
+    }
+    m->dStep = (GraphProp *)p;
+  }
 #undef MTRY
   host_reset(m);
   *out = m;
@@ -367,9 +400,10 @@ extern "C" int dlp_multi_batch_stats(dlp_multi *m, int g, int b, dlp_batch_stats
   }
   int rc = dlp_ctx_fill_stats(c, b, out);
   if (rc != DLP_OK) return rc;
-  /* the graph's batch ran inside the all-graph batch b */
+  /* the graph's batch ran inside the all-graph batch b (no timing when it
+   * ran through dlp_multi_run_next / dlp_multi_serve) */
   float ms = 0.0f;
-  DLP_HIP(hipEventElapsedTime(&ms, m->evStart[b], m->evEnd[b]));
+  if (b < m->maxB && m->ran[b]) DLP_HIP(hipEventElapsedTime(&ms, m->evStart[b], m->evEnd[b]));
   out->timeMs = ms;
   return DLP_OK;
 }
@@ -392,5 +426,147 @@ extern "C" int dlp_multi_device_bytes(const dlp_multi *m, int64_t *bytes) {
   int64_t t = 0;
   for (dlp_context *c : m->ctx) t += c->deviceBytes;
   *bytes = t + (int64_t)sizeof(GraphProp) * m->G * m->maxB;
+  return DLP_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* serving: the next batch of any subset of graphs, and an arrival-driven   */
+/* scheduler over it                                                        */
+/* ------------------------------------------------------------------------ */
+extern "C" int dlp_multi_run_next(dlp_multi *m, const int *graphs, int count) {
+  if (!m || count < 0 || (count > 0 && !graphs)) {
+    dlp_set_error_("dlp_multi_run_next: bad argument");
+    return DLP_ERR_INVALID;
+  }
+  if (count == 0) return DLP_OK;
+  std::vector<char> seen(m->G, 0);
+  for (int i = 0; i < count; ++i) {
+    const int g = graphs[i];
+    if (g < 0 || g >= m->G || seen[g]) {
+      dlp_set_errorf_("dlp_multi_run_next: graph %d out of range or listed twice", g);
+      return DLP_ERR_INVALID;
+    }
+    seen[g] = 1;
+    if (m->ctx[g]->nextBatch >= m->ctx[g]->B) {
+      dlp_set_errorf_("dlp_multi_run_next: graph %d has no batch left", g);
+      return DLP_ERR_STATE;
+    }
+  }
+  DLP_HIP(hipStreamSynchronize(m->stream));  // the staging buffer is free
+  int rc;
+  std::vector<DBatch> bts(count);
+  for (int i = 0; i < count; ++i) {
+    dlp_context *c = m->ctx[graphs[i]];
+    const int b = c->nextBatch;
+    bts[i] = dlp_ctx_make_batch(c, b);
+    int tag = 0;
+    if ((rc = dlp_ctx_batch_pre(c, b, bts[i], &tag)) != DLP_OK) return rc;
+    m->hStep[i] = m->props[graphs[i]];
+  }
+  DLP_HIP(hipMemcpyAsync(m->dStep, m->hStep, sizeof(GraphProp) * count, hipMemcpyHostToDevice,
+                         m->stream));
+  if ((rc = dlp_launch_propagate_multi(m->dStep, count, m->shape, m->ldsWords, m->stream)) !=
+      DLP_OK)
+    return rc;
+  for (int i = 0; i < count; ++i) {
+    dlp_context *c = m->ctx[graphs[i]];
+    if ((rc = dlp_ctx_batch_post(c, c->nextBatch, bts[i])) != DLP_OK) return rc;
+  }
+  return DLP_OK;
+}
+
+namespace {
+
+/* HBM working set of a batch beyond the context's resident state: its
+ * lists, the sparsified subgraph and the sort / reduction scratch of its
+ * inserted vertices (the admission model of the scheduler) */
+int64_t batch_work_bytes(const dlp_context *c, int b) {
+  const int64_t ins = c->insOff[b + 1] - c->insOff[b];
+  const int64_t del = c->delOff[b + 1] - c->delOff[b];
+  const int64_t e = c->eOff[b + 1] - c->eOff[b];
+  return 64 * ins + 8 * del + 36 * e;
+}
+
+}  // namespace
+
+extern "C" int dlp_multi_serve(dlp_multi *m, const dlp_serve_request *reqs, int numReqs,
+                               const dlp_serve_config *cfg, double *completion, int *steps) {
+  if (!m || numReqs < 0 || (numReqs > 0 && (!reqs || !completion)) || !cfg ||
+      cfg->maxSlots < 1) {
+    dlp_set_error_("dlp_multi_serve: bad argument");
+    return DLP_ERR_INVALID;
+  }
+  /* requests of a graph are its next batches, in the given (arrival) order */
+  std::vector<int> need(m->G, 0);
+  for (int i = 0; i < numReqs; ++i) {
+    const int g = reqs[i].graph;
+    if (g < 0 || g >= m->G || (i > 0 && reqs[i].arrival < reqs[i - 1].arrival)) {
+      dlp_set_errorf_("dlp_multi_serve: request %d: bad graph or arrivals not sorted", i);
+      return DLP_ERR_INVALID;
+    }
+    if (m->ctx[g]->nextBatch + ++need[g] > m->ctx[g]->B) {
+      dlp_set_errorf_("dlp_multi_serve: graph %d has fewer batches left than requests", g);
+      return DLP_ERR_INVALID;
+    }
+  }
+  using clock = std::chrono::steady_clock;
+  const clock::time_point t0 = clock::now();
+  auto now = [&] { return std::chrono::duration<double>(clock::now() - t0).count(); };
+  const int slots = cfg->plain ? 1 : cfg->maxSlots;
+  std::vector<int> pending;  // request indices not yet served, in arrival order
+  pending.reserve(numReqs);
+  int nextArrival = 0, nSteps = 0, rc;
+  std::vector<char> blocked(m->G, 0);
+  std::vector<int> admitted, graphs;
+  while (nextArrival < numReqs || !pending.empty()) {
+    const double t = now();
+    while (nextArrival < numReqs && reqs[nextArrival].arrival <= t) pending.push_back(nextArrival++);
+    if (pending.empty()) {
+      std::this_thread::sleep_for(std::chrono::duration<double>(reqs[nextArrival].arrival - t));
+      continue;
+    }
+    /* admission: arrival order, one batch per graph per step, a graph's
+     * batches in order, at most `slots` batches and the HBM budget (the
+     * first admitted batch always fits) */
+    admitted.clear();
+    graphs.clear();
+    std::fill(blocked.begin(), blocked.end(), 0);
+    int64_t used = 0;
+    for (int idx : pending) {
+      const int g = reqs[idx].graph;
+      if (blocked[g]) continue;
+      blocked[g] = 1;  // later requests of g wait for this one
+      if ((int)admitted.size() >= slots) continue;
+      const int64_t w = batch_work_bytes(m->ctx[g], m->ctx[g]->nextBatch);
+      if (!admitted.empty() && cfg->hbmBudgetBytes > 0 && used + w > cfg->hbmBudgetBytes)
+        continue;
+      used += w;
+      admitted.push_back(idx);
+      graphs.push_back(g);
+    }
+    if (cfg->plain) {
+      dlp_context *c = m->ctx[graphs[0]];
+      if ((rc = dlp_context_run_batch(c, c->nextBatch)) != DLP_OK) return rc;
+      DLP_HIP(hipStreamSynchronize(c->stream));
+    } else {
+      if ((rc = dlp_multi_run_next(m, graphs.data(), (int)graphs.size())) != DLP_OK) return rc;
+      DLP_HIP(hipStreamSynchronize(m->stream));
+    }
+    ++nSteps;
+    const double done = now();
+    for (int idx : admitted) completion[idx] = done;
+    std::vector<int> rest;
+    rest.reserve(pending.size());
+    size_t a = 0;
+    for (int idx : pending) {
+      if (a < admitted.size() && admitted[a] == idx) {
+        ++a;
+        continue;
+      }
+      rest.push_back(idx);
+    }
+    pending.swap(rest);
+  }
+  if (steps) *steps = nSteps;
   return DLP_OK;
 }

@@ -145,6 +145,38 @@ extern "C" int dlp_inputs_generate(dlp_inputs **out, int totalNodes, int avgDegr
   return DLP_OK;
 }
 
+extern "C" int dlp_inputs_generate_stream(dlp_inputs **out, int totalNodes, int avgDegree,
+                                          const int *batchSizes, int numBatches,
+                                          unsigned int seed, float groundTruthFraction,
+                                          float deletedFraction) {
+  if (!out || !batchSizes || numBatches <= 0) {
+    dlp_set_error_("dlp_inputs_generate_stream: bad argument");
+    return DLP_ERR_INVALID;
+  }
+  *out = nullptr;
+  dlp_inputs *in = new (std::nothrow) dlp_inputs();
+  if (!in) {
+    dlp_set_error_("dlp_inputs_generate_stream: out of host memory");
+    return DLP_ERR_NOMEM;
+  }
+  bool saved = dlp_host::g_quiet;
+  dlp_host::g_quiet = true;
+  std::vector<BatchData> batches;
+  std::vector<int> sizes(batchSizes, batchSizes + numBatches);
+  bool ok = generateSparseGraph(totalNodes, avgDegree, seed, in->graph, in->trueLabels, "") &&
+            generateStreamBatches(in->graph, in->trueLabels, sizes, seed, groundTruthFraction,
+                                  deletedFraction, batches);
+  dlp_host::g_quiet = saved;
+  if (!ok) {
+    delete in;
+    dlp_set_error_("dlp_inputs_generate_stream: invalid generator arguments");
+    return DLP_ERR_INVALID;
+  }
+  in->flat = flattenBatches(batches);
+  *out = in;
+  return DLP_OK;
+}
+
 extern "C" int dlp_inputs_graph(const dlp_inputs *in, dlp_graph_view *graph,
                                 const float **trueLabels) {
   if (!in || !graph) {

@@ -375,3 +375,95 @@ bool generateBatches(const CSRGraph &graph, const std::vector<float> &trueLabels
   }
   return true;
 }
+
+/* Streaming variant for arrival-driven workloads (no counterpart in the
+ * reference): batch 0 is the balanced GT seed batch of generateBatches;
+ * batch b >= 1 inserts the next sizes[b] vertices of one shuffled order
+ * (fewer once the graph is exhausted), with max(1, int(k * gtFraction)) GT
+ * vertices drawn from the batch and min(int(k * deletedFraction), active)
+ * deletions drawn uniformly from the active non-GT vertices (swap-remove
+ * pool, O(k) per batch).  Edges go to batch max(b_u, b_v) as in
+ * generateBatches. */
+bool generateStreamBatches(const CSRGraph &graph, const std::vector<float> &trueLabels,
+                           const std::vector<int> &sizes, unsigned int seed,
+                           float groundTruthFraction, float deletedFraction,
+                           std::vector<BatchData> &batches) {
+  const int totalNodes = graph.numNodes;
+  const int numBatches = (int)sizes.size();
+  if (numBatches <= 0 || totalNodes <= 0) return false;
+  std::mt19937 rng(seed + 12345);
+  std::vector<int> class0, class1;
+  for (int i = 0; i < totalNodes; ++i) (trueLabels[i] < 0.5f ? class0 : class1).push_back(i);
+  std::shuffle(class0.begin(), class0.end(), rng);
+  std::shuffle(class1.begin(), class1.end(), rng);
+  const int numGTPerClass = std::max(2, static_cast<int>(totalNodes * groundTruthFraction)) / 2;
+  batches.assign(numBatches, BatchData());
+  std::vector<char> isInitialGT(totalNodes, 0);
+  for (int c = 0; c < 2; ++c) {
+    const std::vector<int> &cls = c ? class1 : class0;
+    for (int i = 0; i < numGTPerClass && i < (int)cls.size(); ++i) {
+      const int v = cls[i];
+      batches[0].insertedNodeIds.push_back(v);
+      batches[0].groundTruthNodeIds.push_back(v);
+      batches[0].groundTruthLabels.push_back(c ? 1.0f : 0.0f);
+      isInitialGT[v] = 1;
+    }
+  }
+  std::vector<int> order;
+  order.reserve(totalNodes);
+  for (int i = 0; i < totalNodes; ++i)
+    if (!isInitialGT[i]) order.push_back(i);
+  std::shuffle(order.begin(), order.end(), rng);
+  size_t next = 0;
+  std::vector<int> pool;                  // active non-GT vertices
+  std::vector<int> poolPos(totalNodes, -1);
+  for (int b = 1; b < numBatches; ++b) {
+    BatchData &bt = batches[b];
+    bt.batchId = b;
+    const int k = (int)std::min<size_t>(std::max(sizes[b], 0), order.size() - next);
+    bt.insertedNodeIds.assign(order.begin() + next, order.begin() + next + k);
+    next += k;
+    if (k > 0) {
+      const int numGT = std::max(1, static_cast<int>(k * groundTruthFraction));
+      std::vector<int> shuffled = bt.insertedNodeIds;
+      std::shuffle(shuffled.begin(), shuffled.end(), rng);
+      for (int j = 0; j < numGT && j < k; ++j) {
+        const int v = shuffled[j];
+        bt.groundTruthNodeIds.push_back(v);
+        bt.groundTruthLabels.push_back(trueLabels[v]);
+      }
+      std::vector<int> gtSorted = bt.groundTruthNodeIds;
+      std::sort(gtSorted.begin(), gtSorted.end());
+      for (int v : bt.insertedNodeIds)
+        if (!std::binary_search(gtSorted.begin(), gtSorted.end(), v)) {
+          poolPos[v] = (int)pool.size();
+          pool.push_back(v);
+        }
+    }
+    const int numDel = std::min(static_cast<int>(k * deletedFraction), (int)pool.size());
+    for (int j = 0; j < numDel; ++j) {
+      const int pick = std::uniform_int_distribution<int>(0, (int)pool.size() - 1)(rng);
+      const int v = pool[pick];
+      bt.deletedNodeIds.push_back(v);
+      pool[pick] = pool.back();
+      poolPos[pool[pick]] = pick;
+      pool.pop_back();
+      poolPos[v] = -1;
+    }
+  }
+  std::vector<int> vertexBatch(totalNodes, -1);
+  for (int b = 0; b < numBatches; ++b)
+    for (int v : batches[b].insertedNodeIds) vertexBatch[v] = b;
+  for (int u = 0; u < totalNodes; ++u) {
+    if (vertexBatch[u] < 0) continue;
+    for (int e = graph.rowPtr[u]; e < graph.rowPtr[u + 1]; ++e) {
+      const int v = graph.colInd[e];
+      if (v < 0 || vertexBatch[v] < 0 || u >= v) continue;
+      const int ab = std::max(vertexBatch[u], vertexBatch[v]);
+      batches[ab].newEdgeSrc.push_back(u);
+      batches[ab].newEdgeDst.push_back(v);
+      batches[ab].newEdgeWeight.push_back(graph.values[e]);
+    }
+  }
+  return true;
+}

@@ -105,6 +105,23 @@ def generate(totalNodes: int, avgDegree: int, numBatches: int, seed: int = 42,
     h = C.c_void_p()
     check(lib.dlp_inputs_generate(C.byref(h), totalNodes, avgDegree, numBatches, seed,
                                   gtFraction, deletedFraction))
+    return _inputs_from_handle(lib, h)
+
+
+def generate_stream(totalNodes: int, avgDegree: int, batchSizes, seed: int = 42,
+                    gtFraction: float = 0.01, deletedFraction: float = 0.09) -> Inputs:
+    """The reference graph generator with batches of the given sizes
+    (batchSizes[0] is ignored: batch 0 is the GT seed batch)."""
+    lib = load()
+    sizes = np.ascontiguousarray(batchSizes, np.int32)
+    h = C.c_void_p()
+    check(lib.dlp_inputs_generate_stream(C.byref(h), totalNodes, avgDegree,
+                                         _ptr(sizes, C.c_int), len(sizes), seed, gtFraction,
+                                         deletedFraction))
+    return _inputs_from_handle(lib, h)
+
+
+def _inputs_from_handle(lib, h) -> Inputs:
     try:
         g = _lib.GraphView()
         tl = _lib.f32p()
@@ -386,6 +403,25 @@ class Multi:
         v = C.c_int64()
         check(self._lib.dlp_multi_device_bytes(self._h, C.byref(v)))
         return v.value
+
+    def run_next(self, graphs):
+        """The next batch of each listed graph, together."""
+        arr = np.ascontiguousarray(graphs, np.int32)
+        check(self._lib.dlp_multi_run_next(self._h, _ptr(arr, C.c_int), len(arr)))
+
+    def serve(self, requests, maxSlots: int = 64, hbmBudgetBytes: int = 0,
+              plain: bool = False):
+        """requests: sequence of (graph, arrival seconds), sorted by arrival.
+        Returns (completion seconds per request, steps)."""
+        n = len(requests)
+        reqs = (_lib.ServeRequest * max(n, 1))(*[_lib.ServeRequest(int(g), float(t))
+                                                  for g, t in requests])
+        cfg = _lib.ServeConfig(maxSlots, hbmBudgetBytes, 1 if plain else 0)
+        done = np.zeros(max(n, 1), np.float64)
+        steps = C.c_int()
+        check(self._lib.dlp_multi_serve(self._h, reqs, n, C.byref(cfg),
+                                        _ptr(done, C.c_double), C.byref(steps)))
+        return done[:n], steps.value
 
 
 def batched_workload(numGraphs: int = 256, numBatches: int = 10, seed: int = 42,

@@ -605,6 +605,47 @@ int FN(run_batch)(STATE *st, int b) {
   return 0;
 }
 
+/* IrLP full recompute (validation.cu:47-213): every ACTIVE non-GT vertex
+ * takes the weighted average of its non-deleted neighbours each iteration
+ * (same 256-slot row sum), GT vertices are set to their class, until no
+ * label moves by more than delta; final labels in `labels`.  Returns
+ * iterationsUsed. */
+int FN(irlp)(const int *rowPtr, const int *colInd, const float *values, real *labels,
+             const int *status, const int *gtClass, int numNodes, float delta,
+             int maxIterations) {
+  const size_t N1 = (size_t)(numNodes ? numNodes : 1);
+  real *buf = (real *)malloc(sizeof(real) * N1);
+  memcpy(buf, labels, sizeof(real) * (size_t)numNodes); /* :157-159 */
+  real *old = labels, *nw = buf;
+  const real d = (real)delta;
+  int iterationsUsed = 0;
+  for (int iter = 0; iter < maxIterations; ++iter) {
+    int changed = 0;
+    for (int u = 0; u < numNodes; ++u) { /* irLPKernel (:47-126) */
+      if (gtClass[u] >= 0 || status[u] != V_ACTIVE) {
+        nw[u] = old[u];
+        continue;
+      }
+      real W, S;
+      row_sums(colInd, values, rowPtr[u], rowPtr[u + 1], status, old, &W, &S);
+      real nl = (W > (real)1e-10f) ? S / W : old[u];
+      nl = clamp01(nl);
+      nw[u] = nl;
+      if (absr(nl - old[u]) > d) changed = 1;
+    }
+    for (int v = 0; v < numNodes; ++v) { /* irLPEnforceGTKernel (:131-137) */
+      if (gtClass[v] == 0) nw[v] = (real)0;
+      else if (gtClass[v] == 1) nw[v] = (real)1;
+    }
+    real *tmp = old; old = nw; nw = tmp; /* :186-189 */
+    iterationsUsed = iter + 1;
+    if (!changed) break; /* :192-197 */
+  }
+  if (old != labels) memcpy(labels, old, sizeof(real) * (size_t)numNodes); /* :200-205 */
+  free(buf);
+  return iterationsUsed;
+}
+
 int FN(nnz)(const STATE *st) { return st->nnz; }
 
 void FN(get_csr)(const STATE *st, int *rowPtr, int *colInd, float *values) {

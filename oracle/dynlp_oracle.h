@@ -55,6 +55,9 @@ extern "C" {
                      int numNodes, float delta, int maxIterations,              \
                      int *frontierSizes, int maxFrontierSizes,                  \
                      int *numFrontierSizes);                                    \
+  int PFX##irlp(const int *rowPtr, const int *colInd, const float *values,      \
+                 REAL *labels, const int *status, const int *gtClass,           \
+                 int numNodes, float delta, int maxIterations);                 \
   /* whole batch loop */                                                        \
   PFX##state *PFX##create(                                                      \
       int numNodes, const int *rowPtr, const int *colInd, const float *values,  \

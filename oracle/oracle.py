@@ -50,6 +50,7 @@ def load() -> C.CDLL:
                                                C.c_int, i32p]),
             "propagate": (C.c_int, [i32p, i32p, f32p, rp, rp, i32p, i32p, i32p, C.c_int,
                                     C.c_float, C.c_int, i32p, C.c_int, i32p]),
+            "irlp": (C.c_int, [i32p, i32p, f32p, rp, i32p, i32p, C.c_int, C.c_float, C.c_int]),
             "create": (C.c_void_p, [C.c_int, i32p, i32p, f32p, f32p, C.c_int, i64p, i32p, i64p,
                                     i32p, f32p, i64p, i32p, i64p, i32p, i32p, f32p, C.c_float,
                                     C.c_float, C.c_int]),
@@ -277,3 +278,57 @@ def propagate(rowPtr, colInd, values, labels, status, gtClass, isAffected, delta
                                          _p(status), _p(gtClass), _p(isAffected), N, delta,
                                          maxIterations, _p(fs), cap, _p(nfs))
     return labels, buf, isAffected, int(it), fs[:min(int(nfs[0]), cap)]
+
+
+def irlp(rowPtr, colInd, values, labels, status, gtClass, delta, maxIterations, fp64=False):
+    """IrLP full recompute (validation.cu:47-213): returns (labels, iterationsUsed)."""
+    lib = load()
+    pfx = "orc64_" if fp64 else "orc32_"
+    real = np.float64 if fp64 else np.float32
+    labels = np.array(labels, real)
+    rowPtr = np.ascontiguousarray(rowPtr, np.int32)
+    it = getattr(lib, pfx + "irlp")(_p(rowPtr), _p(np.ascontiguousarray(colInd, np.int32)),
+                                     _p(np.ascontiguousarray(values, np.float32)), _p(labels),
+                                     _p(np.ascontiguousarray(status, np.int32)),
+                                     _p(np.ascontiguousarray(gtClass, np.int32)),
+                                     int(rowPtr.shape[0] - 1), delta, maxIterations)
+    return labels, int(it)
+
+
+def irlp_final_state(inputs, numBatches=None):
+    """The graph IrLP runs on (main.cu:124-187): status after replaying every
+    batch (inserted -> ACTIVE, then deleted -> DELETED, then GT), labels 0.5
+    except GT, and the CSR of non-deleted inserted vertices filtered to
+    non-deleted inserted neighbours.  Returns (rowPtr, colInd, values,
+    status, gtClass, labels)."""
+    N = inputs.numNodes
+    B = inputs.numBatches if numBatches is None else numBatches
+    status = np.full(N, 1, np.int32)   # DELETED
+    gt = np.full(N, -1, np.int32)
+    labels = np.full(N, 0.5, np.float32)
+    batchOf = np.full(N, -1, np.int32)
+    for b in range(B):
+        bt = inputs.batch(b)
+        status[bt["ins"]] = 0
+        status[bt["dels"]] = 1
+        status[bt["gtIds"]] = 2
+        gt[bt["gtIds"]] = (bt["gtLab"] >= 0.5).astype(np.int32)
+        labels[bt["gtIds"]] = bt["gtLab"]
+        batchOf[bt["ins"]] = b
+    keep = (status != 1) & (batchOf >= 0)
+    rp, ci, vv = inputs.rowPtr, inputs.colInd, inputs.values
+    rowPtr = [0]
+    cols, vals = [], []
+    for u in range(N):
+        if keep[u]:
+            seg = slice(rp[u], rp[u + 1])
+            c = ci[seg]
+            ok = (c >= 0) & keep[np.maximum(c, 0)]
+            cols.append(c[ok])
+            vals.append(vv[seg][ok])
+            rowPtr.append(rowPtr[-1] + int(ok.sum()))
+        else:
+            rowPtr.append(rowPtr[-1])
+    colInd = np.concatenate(cols).astype(np.int32) if cols else np.zeros(0, np.int32)
+    values = np.concatenate(vals).astype(np.float32) if vals else np.zeros(0, np.float32)
+    return np.array(rowPtr, np.int32), colInd, values, status, gt, labels

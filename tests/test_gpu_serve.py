@@ -1,0 +1,71 @@
+"""Serving: arrival-driven continuous batching over tenant graphs.  Every
+tenant's result must be the same served alone (its own context, one batch
+at a time) and in the mix, bit for bit."""
+import numpy as np
+import pytest
+
+import labelpropcuda_amd as L
+
+pytestmark = pytest.mark.gpu
+
+
+def bits(a):
+    return np.ascontiguousarray(a, np.float32).view(np.int32)
+
+
+@pytest.fixture(scope="module")
+def tenants():
+    rng = np.random.default_rng(5)
+    out = []
+    for t in range(6):
+        n = int(rng.integers(3000, 30000))
+        nb = int(rng.integers(3, 7))
+        sizes = [0] + [int(x) for x in np.rint(np.exp(rng.uniform(np.log(50), np.log(4000), nb)))]
+        out.append(L.generate_stream(n, 5, sizes, 1042 + t))
+    return out
+
+
+def alone(tenants, maxIter):
+    res = []
+    for inp in tenants:
+        with L.Context(inp, L.make_config(inp, maxIterations=maxIter)) as ctx:
+            ctx.run_all()
+            res.append((ctx.labels(), [ctx.stats(b) for b in range(inp.numBatches)]))
+    return res
+
+
+def requests(tenants, rng, spread):
+    """Every batch after the seed batch of every tenant, arrival times
+    uniform in [0, spread) seconds, sorted; a tenant's batches in order."""
+    reqs = []
+    for t, inp in enumerate(tenants):
+        times = np.sort(rng.uniform(0, spread, inp.numBatches - 1))
+        reqs += [(t, float(x)) for x in times]
+    return sorted(reqs, key=lambda r: r[1])
+
+
+@pytest.mark.parametrize("slots,budget,plain,spread", [
+    (64, 0, False, 0.0),        # everything at once: maximal mixing
+    (3, 0, False, 0.05),        # few slots, staggered arrivals
+    (64, 4096, False, 0.0),     # tiny HBM budget: one batch per step
+    (1, 0, True, 0.0),          # plain: one batch at a time, single-graph path
+])
+def test_serving_same_alone_and_in_mix(gpu_available, tenants, slots, budget, plain, spread):
+    ref = alone(tenants, 3000)
+    cfgs = [L.make_config(t, maxIterations=3000) for t in tenants]
+    rng = np.random.default_rng(11)
+    with L.Multi(tenants, cfgs) as m:
+        m.reset()
+        m.run_next(list(range(len(tenants))))  # seed batches: tenants become resident
+        reqs = requests(tenants, rng, spread)
+        done, steps = m.serve(reqs, maxSlots=slots, hbmBudgetBytes=budget, plain=plain)
+        m.synchronize()
+        assert np.all(done >= np.array([r[1] for r in reqs]))
+        if plain or budget:
+            assert steps == len(reqs)
+        for t, inp in enumerate(tenants):
+            assert np.array_equal(bits(m.labels(t)), bits(ref[t][0])), f"tenant {t}"
+            for b in range(inp.numBatches):
+                s = m.stats(t, b)
+                assert (s["iterations"], s["updates"]) == \
+                    (ref[t][1][b]["iterations"], ref[t][1][b]["updates"]), f"tenant {t} b {b}"
