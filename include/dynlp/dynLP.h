@@ -13,6 +13,7 @@
  *   launchConnectedComponents     <- connectedComponents.h:50-54
  *   launchLabelInitialization     <- labelInitialization.h:54-60
  *   launchIterativePropagation    <- iterativePropagation.h:73-80
+ *   launchIrLP, runIrLPValidation <- validation.h (validation.cu:144-276)
  *   generateSparseGraph           <- generateSparseGraph.h:42-44
  *   generateBatches               <- generateBatches.h:56-59
  *   parseArguments .. writeResultsCSV <- utils.h:41-79
@@ -92,6 +93,21 @@ inline void launchIterativePropagation(
       d_gtClass, d_isAffected, numNodes, delta, maxIterations,
       &iterationsUsed, stream));
 }
+
+/* ---- IrLP full-recompute baseline ---------------------------------------- */
+inline void launchIrLP(const int *d_rowPtr, const int *d_colInd, const float *d_values,
+                       float *d_labels, const int *d_vertexStatus, const int *d_gtClass,
+                       int numNodes, float delta, int maxIterations, int &iterationsUsed,
+                       void *stream) {
+  DLP_CHECK(dlp_irlp(d_rowPtr, d_colInd, d_values, d_labels, d_vertexStatus, d_gtClass,
+                     numNodes, delta, maxIterations, &iterationsUsed, stream));
+}
+
+/* Host arrays in, labels in / out; prints the reference's warning and
+ * returns true for an empty graph. */
+bool runIrLPValidation(const CSRGraph &activeGraph, const std::vector<int> &h_vertexStatus,
+                       const std::vector<int> &h_gtClass, std::vector<float> &h_labels,
+                       int numNodes, float delta, int maxIterations, int &iterationsUsed);
 
 /* ---- synthetic inputs (host) --------------------------------------------- */
 bool generateSparseGraph(int totalNodes, int avgDegree, unsigned int seed,

@@ -407,6 +407,70 @@ int dlp_copy_to_host(void *dst, const void *src, size_t bytes);
  * both pointers are 16-byte aligned); synchronous */
 int dlp_copy_device(void *dst, const void *src, size_t bytes);
 
+/* ====================================================================== */
+/* 6. IrLP: the full-recompute baseline (validation.h, main.cu:120-217)  */
+/* ====================================================================== */
+
+/* Replaces launchIrLP (validation.h, validation.cu:144-205).  Device
+ * pointers; `labels` in / out.  Each iteration sets every vertex that is
+ * ACTIVE and not ground truth (gtClass < 0) to the clamped weighted mean of
+ * its non-deleted neighbours' labels (its old label when the weight sum is
+ * <= 1e-10f), keeps every other label, then forces gtClass 0 / 1 vertices
+ * to 0 / 1.  Stops after the first iteration in which no such label moved
+ * by more than delta, or after maxIterations; iterationsUsed counts the
+ * iterations run.  Same floats as the reference.  The second label buffer
+ * and the changed flags are library workspace (the reference allocates and
+ * frees them per call).  Synchronises `stream`. */
+int dlp_irlp(const int *rowPtr, const int *colInd, const float *values, float *labels,
+             const int *vertexStatus, const int *gtClass, int numNodes, float delta,
+             int maxIterations, int *iterationsUsed, void *stream);
+
+/* Launch shape of the IrLP sweep (no counterpart in the reference). */
+typedef struct dlp_irlp_tuning {
+  int groupLanes; /* lanes per vertex: 4, 8, 16, 32 or 64                 */
+  int blocks;     /* 256-thread blocks per sweep; <= 0: one vertex group
+                     per lane group, at most 8192 blocks                   */
+  int firstChunk; /* iterations queued before the first convergence check;
+                     each later chunk doubles up to maxChunk (<= 256)      */
+  int maxChunk;
+} dlp_irlp_tuning;
+void dlp_irlp_tuning_default(dlp_irlp_tuning *t);
+
+/* dlp_irlp with a launch shape (nullable) and the stream time of the
+ * iteration loop in *kernelMs (nullable). */
+int dlp_irlp_ex(const int *rowPtr, const int *colInd, const float *values, float *labels,
+                const int *vertexStatus, const int *gtClass, int numNodes, float delta,
+                int maxIterations, int *iterationsUsed, void *stream,
+                const dlp_irlp_tuning *tuning, float *kernelMs);
+
+/* Replaces runIrLPValidation (validation.h, validation.cu:210-276): host
+ * CSR of the active graph, host status / gtClass, host labels in / out.
+ * *skipped = 1 (labels untouched, iterationsUsed 0) when numNodes <= 0 or
+ * the graph has no entries, where the reference prints a warning and
+ * returns true. */
+int dlp_irlp_validation(const dlp_graph_view *activeGraph, const int *vertexStatus,
+                        const int *gtClass, float *labels, int numNodes, float delta,
+                        int maxIterations, int *iterationsUsed, int *skipped);
+
+typedef struct dlp_irlp_result {
+  int iterations;      /* iterationsUsed                                   */
+  int skipped;         /* empty final active graph                         */
+  int activeEdges;     /* entries of the final active CSR                  */
+  float timeMs;        /* wall time of the validation call (upload, run,
+                          download), the span main.cu:177-197 times        */
+  float accuracy;      /* over non-deleted, non-ground-truth vertices       */
+} dlp_irlp_result;
+
+/* The reference's --validate step (main.cu:124-217): replays every batch
+ * to the final vertex status, ground-truth classes and initial labels (0.5,
+ * ground truth its label), builds the active CSR (entries of the full graph
+ * between inserted, non-deleted vertices), runs dlp_irlp_validation with
+ * cfg->delta / cfg->maxIterations and scores the result against trueLabels.
+ * irlpLabels: numNodes floats. */
+int dlp_irlp_baseline(const dlp_config *cfg, const dlp_graph_view *fullGraph,
+                      const float *trueLabels, const dlp_batch_view *batches,
+                      float *irlpLabels, dlp_irlp_result *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -155,6 +155,25 @@ class PropProfile(C.Structure):
     ]
 
 
+class IrlpTuning(C.Structure):
+    _fields_ = [
+        ("groupLanes", C.c_int),
+        ("blocks", C.c_int),
+        ("firstChunk", C.c_int),
+        ("maxChunk", C.c_int),
+    ]
+
+
+class IrlpResult(C.Structure):
+    _fields_ = [
+        ("iterations", C.c_int),
+        ("skipped", C.c_int),
+        ("activeEdges", C.c_int),
+        ("timeMs", C.c_float),
+        ("accuracy", C.c_float),
+    ]
+
+
 # name -> (restype, argtypes)
 _VP = C.c_void_p
 SIGNATURES = {
@@ -223,6 +242,14 @@ SIGNATURES = {
                                              C.POINTER(C.c_int), C.c_int, C.c_uint, C.c_float,
                                              C.c_float]),
     "dlp_copy_device": (C.c_int, [_VP, _VP, C.c_size_t]),
+    "dlp_irlp": (C.c_int, [_VP, _VP, _VP, _VP, _VP, _VP, C.c_int, C.c_float, C.c_int, i32p, _VP]),
+    "dlp_irlp_tuning_default": (None, [C.POINTER(IrlpTuning)]),
+    "dlp_irlp_ex": (C.c_int, [_VP, _VP, _VP, _VP, _VP, _VP, C.c_int, C.c_float, C.c_int, i32p, _VP,
+                              C.POINTER(IrlpTuning), f32p]),
+    "dlp_irlp_validation": (C.c_int, [C.POINTER(GraphView), i32p, i32p, f32p, C.c_int, C.c_float,
+                                      C.c_int, i32p, i32p]),
+    "dlp_irlp_baseline": (C.c_int, [C.POINTER(Config), C.POINTER(GraphView), f32p,
+                                    C.POINTER(BatchView), f32p, C.POINTER(IrlpResult)]),
     "dlp_device_malloc": (C.c_int, [C.POINTER(_VP), C.c_size_t]),
     "dlp_device_free": (C.c_int, [_VP]),
     "dlp_copy_to_device": (C.c_int, [_VP, _VP, C.c_size_t]),

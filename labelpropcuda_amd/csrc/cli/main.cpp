@@ -3,8 +3,9 @@
  * bin/dynLP, src/main.cu:53-298): same flags, same output files
  * (dynlp_labels.txt, results.csv, graph/csr*.txt) and the same summary.
  *
- * --validate (the IrLP full-recompute baseline, main.cu:120-220) is not part
- * of the dynamic-update path; the flag is accepted and reported as skipped.
+ * --validate runs the IrLP full-recompute baseline on the final state of the
+ * batch stream (main.cu:120-217, dlp_irlp_baseline), writes irlp_labels.txt
+ * and adds the IrLP time, accuracy and speedup to the summary.
  */
 #include <iomanip>
 #include <iostream>
@@ -41,8 +42,21 @@ int main(int argc, char *argv[]) {
     std::cout << "Error: DynLP execution failed.\n";
     return 1;
   }
-  if (config.validate)
-    std::cout << "IrLP baseline (--validate) is not part of the batch-update path; skipped.\n\n";
+  std::vector<float> irlpLabels;
+  dlp_irlp_result irlp{};
+  if (config.validate) {
+    std::cout << "Running IrLP baseline for validation...\n";
+    FlatBatches flat = flattenBatches(batches);
+    dlp_batch_view bv = flat.view();
+    dlp_graph_view gv = graphView(fullGraph);
+    dlp_config c = toCConfig(config);
+    irlpLabels.assign(config.totalNodes, 0.5f);
+    DLP_CHECK(dlp_irlp_baseline(&c, &gv, trueLabels.data(), &bv, irlpLabels.data(), &irlp));
+    if (irlp.skipped) std::cout << "Warning: IrLP validation skipped (empty graph).\n";
+    std::cout << "IrLP baseline: " << irlp.iterations << " iterations, " << irlp.timeMs
+              << " ms, accuracy " << std::fixed << std::setprecision(2)
+              << (irlp.accuracy * 100.0f) << "%\n\n";
+  }
 
   if (!writeLabelsToFile(config.outputDir + "/dynlp_labels.txt", labels.data(), config.totalNodes)) {
     std::cout << "Error: Could not write DynLP labels.\n";
@@ -51,6 +65,12 @@ int main(int argc, char *argv[]) {
   if (!writeResultsCSV(config.outputDir + "/results.csv", batchTimes.data(), batchAccuracy.data(),
                        config.numBatches)) {
     std::cout << "Error: Could not write results CSV.\n";
+    return 1;
+  }
+  if (config.validate &&
+      !writeLabelsToFile(config.outputDir + "/irlp_labels.txt", irlpLabels.data(),
+                         config.totalNodes)) {
+    std::cout << "Error: Could not write IrLP labels.\n";
     return 1;
   }
 
@@ -65,8 +85,17 @@ int main(int argc, char *argv[]) {
             << "  Average accuracy:     " << std::fixed << std::setprecision(2) << (avgAcc * 100.0f)
             << "%\n"
             << "  Final batch accuracy: " << std::fixed << std::setprecision(2)
-            << (batchAccuracy.back() * 100.0f) << "%\n"
-            << "============================================================\n\n"
+            << (batchAccuracy.back() * 100.0f) << "%\n";
+  if (config.validate) {
+    const float speedup = irlp.timeMs > 0.0f ? irlp.timeMs / totalTime : 0.0f;
+    std::cout << "  IrLP baseline time:   " << std::fixed << std::setprecision(2) << irlp.timeMs
+              << " ms\n"
+              << "  IrLP accuracy:        " << std::fixed << std::setprecision(2)
+              << (irlp.accuracy * 100.0f) << "%\n"
+              << "  Speedup vs IrLP:      " << std::fixed << std::setprecision(1) << speedup
+              << "x\n";
+  }
+  std::cout << "============================================================\n\n"
             << "Per-batch results:\n";
   for (int b = 0; b < config.numBatches; ++b)
     std::cout << "  Batch " << std::setw(2) << b << ": " << std::fixed << std::setprecision(2)
@@ -74,7 +103,8 @@ int main(int argc, char *argv[]) {
               << std::setprecision(2) << (batchAccuracy[b] * 100.0f) << "%\n";
   std::cout << "\nOutput files written to: " << config.outputDir << "/\n"
             << "  dynlp_labels.txt  - Final predicted labels\n"
-            << "  results.csv       - Per-batch timing and accuracy\n"
-            << "  graph/            - Generated graph in CSR format\n";
+            << "  results.csv       - Per-batch timing and accuracy\n";
+  if (config.validate) std::cout << "  irlp_labels.txt   - IrLP baseline labels\n";
+  std::cout << "  graph/            - Generated graph in CSR format\n";
   return 0;
 }

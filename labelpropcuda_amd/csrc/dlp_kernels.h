@@ -75,6 +75,38 @@ int dlp_local_lds_bytes(int *bytes);
  * padded to 8 records) */
 int dlp_local_shape_count();
 
+/* ------------------------------------------------------------------------ */
+/* IrLP full recompute (dlp_irlp.hip)                                       */
+/* ------------------------------------------------------------------------ */
+struct IrlpArgs {
+  const int *rowPtr;  /* standard CSR, numNodes + 1                          */
+  const int *colInd;
+  const float *values;
+  const int *status;
+  const int *gtClass;
+  float *lab0;        /* the caller's labels: read by even iterations        */
+  float *lab1;        /* second buffer, numNodes floats                       */
+  int *flags;         /* kIrlpMaxChunk + 1 words                              */
+  int numNodes;
+  float delta;
+};
+constexpr int kIrlpMaxChunk = 256;
+struct IrlpLoopConfig {
+  int groupLanes = 8;  /* lanes per vertex, 4 .. 64                           */
+  int blocks = 0;      /* <= 0: enough 256-thread blocks for one vertex group
+                          each, at most 8192                                  */
+  int firstChunk = 8;  /* iterations queued before the first host check; each
+                          later chunk doubles, up to maxChunk                 */
+  int maxChunk = 64;
+};
+int dlp_irlp_blocks_default(int numNodes, int groupLanes);
+/* Runs IrLP on device buffers until an iteration changes nothing or
+ * maxIterations; the final labels end in a.lab0.  hostFlags: pinned, at
+ * least cfg.maxChunk + 1 words.  kernelMs (nullable): stream time of the
+ * whole loop.  Synchronises s. */
+int dlp_run_irlp(const IrlpArgs &a, int maxIterations, const IrlpLoopConfig &cfg, int *hostFlags,
+                 hipStream_t s, int *iterationsUsed, float *kernelMs);
+
 /* one graph of a multi-graph propagation launch */
 struct GraphProp {
   PropArgs a;

@@ -24,6 +24,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include "dynlp/dynlp_c.h"
+
 namespace dlp {
 
 template <class Pos>
@@ -130,6 +132,27 @@ __device__ __forceinline__ float new_label(float W, float S, float oldLabel) {
   float nl = (W > 1e-10f) ? S / W : oldLabel;
   return fminf(fmaxf(nl, 0.0f), 1.0f);
 }
+
+/* row position p of the CSR row that starts at `beg`: a soft-deleted entry
+ * (negative column) or one pointing at a deleted vertex adds nothing
+ * (iterativePropagation.cu:126-137, validation.cu:82-91) */
+struct GlobalPos {
+  const int *col;
+  const float *val;
+  const float *F;
+  const int *status;
+  int beg;
+  __device__ void operator()(int p, float &w, float &s) const {
+    const int v = col[beg + p];
+    if (v >= 0 && !(status && status[v] == DLP_VERTEX_DELETED)) {
+      w = val[beg + p];
+      s = w * F[v];
+    } else {
+      w = 0.0f;
+      s = 0.0f;
+    }
+  }
+};
 
 }  // namespace dlp
 

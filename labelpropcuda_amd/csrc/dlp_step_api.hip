@@ -26,6 +26,7 @@ struct Workspace {
   void *coreGenBuf = nullptr; /* the slot buffer localGen refers to */
   size_t coreGenBytes = 0;
   int ldsBytes = 0;
+  int *pinFlags = nullptr; /* pinned changed flags of the IrLP loop */
 };
 
 std::mutex g_wsMutex;
@@ -378,4 +379,126 @@ extern "C" int dlp_iterative_propagation(const int *rowPtr, const int *colInd,
   if (rc != DLP_OK) return rc;
   DLP_HIP(hipStreamSynchronize(s));
   return DLP_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* IrLP full recompute                                                      */
+/* ------------------------------------------------------------------------ */
+extern "C" void dlp_irlp_tuning_default(dlp_irlp_tuning *t) {
+  if (!t) return;
+  const IrlpLoopConfig d;
+  t->groupLanes = d.groupLanes;
+  t->blocks = d.blocks;
+  t->firstChunk = d.firstChunk;
+  t->maxChunk = d.maxChunk;
+}
+
+extern "C" int dlp_irlp_ex(const int *rowPtr, const int *colInd, const float *values,
+                           float *labels, const int *vertexStatus, const int *gtClass,
+                           int numNodes, float delta, int maxIterations, int *iterationsUsed,
+                           void *stream, const dlp_irlp_tuning *tuning, float *kernelMs) {
+  if (!iterationsUsed) {
+    dlp_set_error_("dlp_irlp: NULL iterationsUsed");
+    return DLP_ERR_INVALID;
+  }
+  *iterationsUsed = 0;
+  if (kernelMs) *kernelMs = 0.0f;
+  if (numNodes < 0 || (numNodes > 0 && (!rowPtr || !labels || !vertexStatus || !gtClass))) {
+    dlp_set_error_("dlp_irlp: bad argument");
+    return DLP_ERR_INVALID;
+  }
+  if (numNodes == 0 || maxIterations <= 0) return DLP_OK; /* validation.cu:168 loop not entered */
+  IrlpLoopConfig cfg;
+  if (tuning) {
+    cfg.groupLanes = tuning->groupLanes;
+    cfg.blocks = tuning->blocks;
+    cfg.firstChunk = tuning->firstChunk;
+    cfg.maxChunk = tuning->maxChunk;
+  }
+  std::lock_guard<std::mutex> lock(g_wsMutex);
+  Workspace *w;
+  int rc = ws_open(&w);
+  if (rc != DLP_OK) return rc;
+  if (!w->pinFlags)
+    DLP_HIP(hipHostMalloc((void **)&w->pinFlags, sizeof(int) * (kIrlpMaxChunk + 1),
+                          hipHostMallocDefault));
+  WS_BUF(float, buffer, 24, numNodes);
+  WS_BUF(int, flags, 25, kIrlpMaxChunk + 1);
+  hipStream_t s = (hipStream_t)stream;
+  /* the second buffer starts as a copy of the labels (validation.cu:157-159) */
+  if ((rc = dlp_k_copy_float(buffer, labels, numNodes, s)) != DLP_OK) return rc;
+  IrlpArgs a;
+  a.rowPtr = rowPtr;
+  a.colInd = colInd;
+  a.values = values;
+  a.status = vertexStatus;
+  a.gtClass = gtClass;
+  a.lab0 = labels;
+  a.lab1 = buffer;
+  a.flags = flags;
+  a.numNodes = numNodes;
+  a.delta = delta;
+  return dlp_run_irlp(a, maxIterations, cfg, w->pinFlags, s, iterationsUsed, kernelMs);
+}
+
+extern "C" int dlp_irlp(const int *rowPtr, const int *colInd, const float *values, float *labels,
+                        const int *vertexStatus, const int *gtClass, int numNodes, float delta,
+                        int maxIterations, int *iterationsUsed, void *stream) {
+  return dlp_irlp_ex(rowPtr, colInd, values, labels, vertexStatus, gtClass, numNodes, delta,
+                     maxIterations, iterationsUsed, stream, nullptr, nullptr);
+}
+
+extern "C" int dlp_irlp_validation(const dlp_graph_view *activeGraph, const int *vertexStatus,
+                                   const int *gtClass, float *labels, int numNodes, float delta,
+                                   int maxIterations, int *iterationsUsed, int *skipped) {
+  if (!activeGraph || !iterationsUsed) {
+    dlp_set_error_("dlp_irlp_validation: NULL argument");
+    return DLP_ERR_INVALID;
+  }
+  *iterationsUsed = 0;
+  if (skipped) *skipped = 0;
+  if (numNodes <= 0 || activeGraph->numEdges <= 0) { /* validation.cu:216-219 */
+    if (skipped) *skipped = 1;
+    return DLP_OK;
+  }
+  if (activeGraph->numNodes != numNodes || !activeGraph->rowPtr || !activeGraph->colInd ||
+      !activeGraph->values || !vertexStatus || !gtClass || !labels) {
+    dlp_set_error_("dlp_irlp_validation: bad argument");
+    return DLP_ERR_INVALID;
+  }
+  const size_t N = (size_t)numNodes, E = (size_t)activeGraph->numEdges;
+  void *buf[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  const size_t bytes[6] = {sizeof(int) * (N + 1), sizeof(int) * E, sizeof(float) * E,
+                           sizeof(float) * N,     sizeof(int) * N, sizeof(int) * N};
+  const void *src[6] = {activeGraph->rowPtr, activeGraph->colInd, activeGraph->values,
+                        labels,              vertexStatus,        gtClass};
+  hipStream_t s = nullptr;
+  int rc = DLP_OK;
+  hipError_t e = hipStreamCreate(&s);
+  for (int i = 0; i < 6 && e == hipSuccess; ++i) e = hipMalloc(&buf[i], bytes[i]);
+  for (int i = 0; i < 6 && e == hipSuccess; ++i)
+    e = hipMemcpyAsync(buf[i], src[i], bytes[i], hipMemcpyHostToDevice, s);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    dlp_set_errorf_("dlp_irlp_validation: HIP error %d (%s)", (int)e, hipGetErrorString(e));
+    rc = DLP_ERR_HIP;
+  }
+  if (rc == DLP_OK)
+    rc = dlp_irlp(static_cast<const int *>(buf[0]), static_cast<const int *>(buf[1]),
+                  static_cast<const float *>(buf[2]), static_cast<float *>(buf[3]),
+                  static_cast<const int *>(buf[4]), static_cast<const int *>(buf[5]), numNodes,
+                  delta, maxIterations, iterationsUsed, s);
+  if (rc == DLP_OK) {
+    e = hipMemcpyAsync(labels, buf[3], bytes[3], hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      dlp_set_errorf_("dlp_irlp_validation: HIP error %d (%s)", (int)e, hipGetErrorString(e));
+      rc = DLP_ERR_HIP;
+    }
+  }
+  for (int i = 0; i < 6; ++i)
+    if (buf[i]) (void)hipFree(buf[i]);
+  if (s) (void)hipStreamDestroy(s);
+  return rc;
 }

@@ -5,7 +5,10 @@
  * runDynLP (dynLP.h:69-74) keeps its signature and its error behaviour
  * (print and exit, graphTypes.h:54-62); the work happens in dlp_run_dynlp.
  */
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <iostream>
 #include <new>
 #include <vector>
 
@@ -91,6 +94,24 @@ bool runDynLP(const DynLPConfig &config, const CSRGraph &graph,
   batchAccuracy.assign(config.numBatches, 0.0f);
   DLP_CHECK(dlp_run_dynlp(&c, &gv, trueLabels.data(), &bv, finalLabels.data(),
                           batchTimes.data(), batchAccuracy.data()));
+  return true;
+}
+
+bool runIrLPValidation(const CSRGraph &activeGraph, const std::vector<int> &h_vertexStatus,
+                       const std::vector<int> &h_gtClass, std::vector<float> &h_labels,
+                       int numNodes, float delta, int maxIterations, int &iterationsUsed) {
+  dlp_graph_view gv = graphView(activeGraph);
+  int skipped = 0;
+  iterationsUsed = 0;
+  if (numNodes > 0 && activeGraph.numEdges > 0 &&
+      ((int)h_vertexStatus.size() < numNodes || (int)h_gtClass.size() < numNodes ||
+       (int)h_labels.size() < numNodes)) {
+    std::fprintf(stderr, "runIrLPValidation: arrays shorter than numNodes\n");
+    std::exit(EXIT_FAILURE);
+  }
+  DLP_CHECK(dlp_irlp_validation(&gv, h_vertexStatus.data(), h_gtClass.data(), h_labels.data(),
+                                numNodes, delta, maxIterations, &iterationsUsed, &skipped));
+  if (skipped) std::cout << "Warning: IrLP validation skipped (empty graph).\n";
   return true;
 }
 

@@ -1,0 +1,193 @@
+/*
+ * dlp_irlp.hip -- IrLP, the full-recompute baseline (validation.cu:47-205).
+ *
+ * Every iteration recomputes the label of every active vertex that is not
+ * ground truth from all of its neighbours, double-buffered, with ground
+ * truth enforced to 0 / 1, until no label moves by more than delta.
+ *
+ * The reference runs one 256-thread block per vertex and one enforcement
+ * kernel per iteration, and copies a changed flag to the host after every
+ * iteration.  Here:
+ *   - one sweep kernel per iteration: G lanes (4 .. 64, a power of two)
+ *     per vertex, grid-stride over consecutive vertices, so the row
+ *     pointers, labels, status and ground-truth class of a wave are read in
+ *     runs and a row's entries are contiguous lanes of one load; ground
+ *     truth enforcement is fused into the same store;
+ *   - the row sums are the reference block's 256-slot tree
+ *     (dlp::group_row_sums), so every label is the same float;
+ *   - the changed flag of iteration i is word i+1 of a small per-chunk
+ *     array; the kernel of iteration i returns at once when word i is zero
+ *     (iteration i-1 changed nothing), so a chunk of iterations is queued
+ *     without a host round trip and the host reads the words once per chunk.
+ *     The first zero word gives the reference's iterationsUsed exactly;
+ *     kernels queued after it write nothing.
+ *
+ * Per iteration over N vertices and E stored entries the algorithmic
+ * traffic is 20 B per vertex (row pointer, ground-truth class, status, old
+ * and new label) and 16 B per entry (column, weight, the neighbour's status
+ * and label).
+ */
+#include <algorithm>
+
+#include "../dlp_kernels.h"
+#include "../dlp_rowsum.h"
+
+namespace {
+
+constexpr int kIrlpThreads = 256;
+
+/* word 0 <- 1 (the iteration before the chunk changed something), words
+ * 1..C <- 0 */
+__global__ void k_irlp_flags(int *flags, int C) {
+  for (int i = threadIdx.x; i <= C; i += blockDim.x) flags[i] = i == 0 ? 1 : 0;
+}
+
+/* iteration `iter` (global index, picks the buffers) as word j of the
+ * chunk */
+template <int G>
+__global__ __launch_bounds__(kIrlpThreads) void k_irlp(IrlpArgs a, int iter, int j) {
+  if (a.flags[j] == 0) return;  // converged before this iteration (grid-uniform)
+  constexpr int GPW = 64 / G;   /* vertices per wave */
+  const float *Fold = (iter & 1) ? a.lab1 : a.lab0;
+  float *Fnew = (iter & 1) ? a.lab0 : a.lab1;
+  const int lane = threadIdx.x & 63;
+  const int gl = lane & (G - 1);
+  const int grp = lane / G;
+  const long long gwave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const long long nwaves = ((long long)gridDim.x * blockDim.x) >> 6;
+  const long long N = a.numNodes;
+  bool changed = false;
+  for (long long base = gwave * GPW; base < N; base += nwaves * GPW) {  // wave-uniform
+    const long long u = base + grp;
+    const bool act = u < N;
+    int gt = -1, st = DLP_VERTEX_DELETED, beg = 0, len = 0;
+    if (act) {
+      gt = a.gtClass[u];
+      st = a.status[u];
+    }
+    /* validation.cu:56-70: ground truth and non-active vertices keep their
+     * label (then :131-137 enforces ground truth) */
+    const bool live = act && gt < 0 && st == DLP_VERTEX_ACTIVE;
+    if (live) {
+      beg = a.rowPtr[u];
+      len = a.rowPtr[u + 1] - beg;
+    }
+    float W, S;
+    dlp::group_row_sums<G>(gl, len, dlp::GlobalPos{a.colInd, a.values, Fold, a.status, beg}, W,
+                           S);
+    if (act && gl == 0) {
+      float nl;
+      if (gt == 0) {
+        nl = 0.0f;
+      } else if (gt == 1) {
+        nl = 1.0f;
+      } else {
+        const float old = Fold[u];
+        nl = old;
+        if (live) {  // validation.cu:107-126
+          nl = dlp::new_label(W, S, old);
+          changed = changed || fabsf(nl - old) > a.delta;
+        }
+      }
+      Fnew[u] = nl;
+    }
+  }
+  if (__any(changed) && lane == 0) a.flags[j + 1] = 1;
+}
+
+template <int G>
+int launch_irlp(const IrlpArgs &a, int iter, int j, int blocks, hipStream_t s) {
+  k_irlp<G><<<blocks, kIrlpThreads, 0, s>>>(a, iter, j);
+  return DLP_OK;
+}
+
+}  // namespace
+
+int dlp_irlp_blocks_default(int numNodes, int groupLanes) {
+  const long long waves = ((long long)numNodes * groupLanes + 63) / 64;
+  const long long blocks = (waves + kIrlpThreads / 64 - 1) / (kIrlpThreads / 64);
+  return (int)std::max<long long>(1, std::min<long long>(blocks, 8192));
+}
+
+int dlp_run_irlp(const IrlpArgs &a, int maxIterations, const IrlpLoopConfig &cfg, int *hostFlags,
+                 hipStream_t s, int *iterationsUsed, float *kernelMs) {
+  *iterationsUsed = 0;
+  if (kernelMs) *kernelMs = 0.0f;
+  if (maxIterations <= 0 || a.numNodes <= 0) return DLP_OK;
+  const int G = cfg.groupLanes;
+  if (G != 4 && G != 8 && G != 16 && G != 32 && G != 64) {
+    dlp_set_errorf_("IrLP: group lanes %d not in {4,8,16,32,64}", G);
+    return DLP_ERR_INVALID;
+  }
+  if (cfg.firstChunk < 1 || cfg.maxChunk < cfg.firstChunk || cfg.maxChunk > kIrlpMaxChunk) {
+    dlp_set_error_("IrLP: bad chunk lengths");
+    return DLP_ERR_INVALID;
+  }
+  const int blocks = cfg.blocks > 0 ? cfg.blocks : dlp_irlp_blocks_default(a.numNodes, G);
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  if (kernelMs) {
+    DLP_HIP(hipEventCreate(&ev[0]));
+    DLP_HIP(hipEventCreate(&ev[1]));
+    DLP_HIP(hipEventRecord(ev[0], s));
+  }
+  int used = -1, it = 0, chunk = cfg.firstChunk;
+  int rc = DLP_OK;
+  while (it < maxIterations && used < 0 && rc == DLP_OK) {
+    const int C = std::min(chunk, maxIterations - it);
+    k_irlp_flags<<<1, 64, 0, s>>>(a.flags, C);
+    for (int j = 0; j < C && rc == DLP_OK; ++j) {
+      switch (G) {
+        case 4: rc = launch_irlp<4>(a, it + j, j, blocks, s); break;
+        case 8: rc = launch_irlp<8>(a, it + j, j, blocks, s); break;
+        case 16: rc = launch_irlp<16>(a, it + j, j, blocks, s); break;
+        case 32: rc = launch_irlp<32>(a, it + j, j, blocks, s); break;
+        default: rc = launch_irlp<64>(a, it + j, j, blocks, s); break;
+      }
+    }
+    if (rc != DLP_OK) break;
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(hostFlags, a.flags, sizeof(int) * (size_t)(C + 1), hipMemcpyDeviceToHost,
+                         s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      dlp_set_errorf_("IrLP: HIP error %d (%s)", (int)e, hipGetErrorString(e));
+      rc = DLP_ERR_HIP;
+      break;
+    }
+    for (int j = 0; j < C; ++j)
+      if (hostFlags[j + 1] == 0) {  // iteration it+j changed nothing (validation.cu:192-196)
+        used = it + j + 1;
+        break;
+      }
+    it += C;
+    chunk = std::min(2 * chunk, cfg.maxChunk);
+  }
+  if (rc == DLP_OK) {
+    if (used < 0) used = maxIterations;
+    /* the last iteration wrote lab1 when its index is even
+     * (validation.cu:199-203) */
+    if (used & 1) {
+      hipError_t e = hipMemcpyAsync(a.lab0, a.lab1, sizeof(float) * (size_t)a.numNodes,
+                                    hipMemcpyDeviceToDevice, s);
+      if (e != hipSuccess) {
+        (void)hipGetLastError();
+        dlp_set_errorf_("IrLP: HIP error %d (%s)", (int)e, hipGetErrorString(e));
+        rc = DLP_ERR_HIP;
+      }
+    }
+    *iterationsUsed = used;
+  }
+  if (ev[0]) {
+    if (rc == DLP_OK) {
+      (void)hipEventRecord(ev[1], s);
+      (void)hipEventSynchronize(ev[1]);
+      (void)hipEventElapsedTime(kernelMs, ev[0], ev[1]);
+    }
+    (void)hipEventDestroy(ev[0]);
+    (void)hipEventDestroy(ev[1]);
+  }
+  if (rc == DLP_OK) DLP_HIP(hipStreamSynchronize(s));
+  return rc;
+}

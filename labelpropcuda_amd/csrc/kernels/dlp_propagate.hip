@@ -36,23 +36,7 @@ namespace {
 /* ------------------------------------------------------------------------ */
 /* multi-block iteration                                                    */
 /* ------------------------------------------------------------------------ */
-struct GlobalPos {
-  const int *col;
-  const float *val;
-  const float *F;
-  const int *status;
-  int beg;
-  __device__ void operator()(int p, float &w, float &s) const {
-    const int v = col[beg + p];
-    if (v >= 0 && !(status && status[v] == DLP_VERTEX_DELETED)) {
-      w = val[beg + p];
-      s = w * F[v];
-    } else {
-      w = 0.0f;
-      s = 0.0f;
-    }
-  }
-};
+using dlp::GlobalPos;
 
 constexpr int kPT = 512;       /* threads of the multi-block kernel          */
 constexpr int kStageQ = 4096;  /* per-block staged next-frontier appends     */

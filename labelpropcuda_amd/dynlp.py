@@ -179,6 +179,76 @@ def run_dynlp(inputs: Inputs, cfg: _lib.Config):
     return labels, times, acc
 
 
+def irlp_tuning(**kw) -> _lib.IrlpTuning:
+    """IrLP launch shape: library defaults, then the given fields."""
+    t = _lib.IrlpTuning()
+    load().dlp_irlp_tuning_default(C.byref(t))
+    for k, v in kw.items():
+        setattr(t, k, int(v))
+    return t
+
+
+def irlp(rowPtr, colInd, values, labels, status, gtClass, delta: float = 1e-4,
+         maxIterations: int = 100000, tuning: _lib.IrlpTuning | None = None):
+    """launchIrLP (validation.cu:144-205) on host arrays, through dlp_irlp_ex:
+    returns (labels, iterationsUsed, loopMs)."""
+    lib = load()
+    rowPtr = np.ascontiguousarray(rowPtr, np.int32)
+    N = int(rowPtr.shape[0] - 1)
+    dev = [_lib.DeviceArray(np.ascontiguousarray(a, t)) for a, t in
+           ((rowPtr, np.int32), (colInd, np.int32), (values, np.float32),
+            (labels, np.float32), (status, np.int32), (gtClass, np.int32))]
+    it = C.c_int(0)
+    ms = C.c_float(0.0)
+    try:
+        check(lib.dlp_irlp_ex(dev[0].ptr, dev[1].ptr, dev[2].ptr, dev[3].ptr, dev[4].ptr,
+                              dev[5].ptr, N, C.c_float(delta), int(maxIterations), C.byref(it),
+                              None, C.byref(tuning) if tuning is not None else None,
+                              C.byref(ms)))
+        out = dev[3].numpy()
+    finally:
+        for d in dev:
+            d.free()
+    return out, it.value, ms.value
+
+
+def irlp_validation(rowPtr, colInd, values, status, gtClass, labels, delta: float = 1e-4,
+                    maxIterations: int = 100000):
+    """runIrLPValidation (validation.cu:210-276) over host arrays: returns
+    (labels, iterationsUsed, skipped)."""
+    lib = load()
+    rowPtr = np.ascontiguousarray(rowPtr, np.int32)
+    colInd = np.ascontiguousarray(colInd, np.int32)
+    values = np.ascontiguousarray(values, np.float32)
+    status = np.ascontiguousarray(status, np.int32)
+    gtClass = np.ascontiguousarray(gtClass, np.int32)
+    out = np.array(labels, np.float32)
+    g = _lib.GraphView()
+    g.numNodes = int(rowPtr.shape[0] - 1)
+    g.numEdges = int(colInd.shape[0])
+    g.rowPtr = _ptr(rowPtr, C.c_int)
+    g.colInd = _ptr(colInd, C.c_int)
+    g.values = _ptr(values, C.c_float)
+    it, skipped = C.c_int(0), C.c_int(0)
+    check(lib.dlp_irlp_validation(C.byref(g), _ptr(status, C.c_int), _ptr(gtClass, C.c_int),
+                                  _ptr(out, C.c_float), g.numNodes, C.c_float(delta),
+                                  int(maxIterations), C.byref(it), C.byref(skipped)))
+    return out, it.value, bool(skipped.value)
+
+
+def irlp_baseline(inputs: Inputs, cfg: _lib.Config):
+    """The --validate step (main.cu:124-217): IrLP on the final state of the
+    batch stream.  Returns (irlpLabels, {iterations, skipped, activeEdges,
+    timeMs, accuracy})."""
+    lib = load()
+    labels = np.zeros(inputs.numNodes, np.float32)
+    res = _lib.IrlpResult()
+    g, bv = inputs.graph_view(), inputs.batch_view()
+    check(lib.dlp_irlp_baseline(C.byref(cfg), C.byref(g), _ptr(inputs.trueLabels, C.c_float),
+                                C.byref(bv), _ptr(labels, C.c_float), C.byref(res)))
+    return labels, {f: getattr(res, f) for f, _ in res._fields_}
+
+
 class TraceBuffers:
     """Host buffers for dlp_trace (the intermediate state of one batch)."""
 
