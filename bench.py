@@ -307,6 +307,93 @@ def serving_main(args) -> int:
     return 0
 
 
+def irlp_main(args) -> int:
+    """IrLP, the reference's full-recompute baseline (--validate), on the
+    final state of the same 10-batch stream, beside the DynLP stream time it
+    is compared with (the reference's "Speedup vs IrLP")."""
+    import labelpropcuda_amd as L
+    inputs = L.generate(args.nodes, args.degree, args.batches, 42)
+    cfg = L.make_config(inputs)
+    N, B = inputs.numNodes, inputs.numBatches
+    with L.Context(inputs, cfg) as ctx:
+        for _ in range(max(args.warmup, 1) + 1):
+            ctx.run_all()
+            ctx.synchronize()
+        dyn_ms = float(sum(ctx.stats(b)["timeMs"] for b in range(B)))
+        dyn_acc = float(ctx.stats(B - 1)["accuracy"])
+
+    # launch shape: lanes per vertex with the shortest loop
+    sweep = {}
+    for g in (4, 8, 16, 32):
+        _, r = L.dynlp.irlp_baseline(inputs, cfg, L.dynlp.irlp_tuning(groupLanes=g))
+        sweep[g] = r["loopMs"]
+    best = min(sweep, key=sweep.get)
+    tuning = L.dynlp.irlp_tuning(groupLanes=best)
+    for _ in range(args.warmup):
+        L.dynlp.irlp_baseline(inputs, cfg, tuning)
+    runs = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        _, r = L.dynlp.irlp_baseline(inputs, cfg, tuning)
+        runs.append(r)
+    wall = time.perf_counter() - t0
+    its = runs[0]["iterations"]
+    assert all(r["iterations"] == its for r in runs), "IrLP iteration count changed"
+    loop_ms = float(np.mean([r["loopMs"] for r in runs]))
+    time_ms = float(np.mean([r["timeMs"] for r in runs]))
+    E = runs[0]["activeEdges"]
+    per_iter_s = loop_ms / 1000.0 / its
+    byts = 20 * N + 16 * E  # per sweep: vertex 20 B, stored entry 16 B
+    achieved = byts / per_iter_s / 1e9
+    value = N * its / (loop_ms / 1000.0)
+    out = {
+        "metric": "IrLP full-recompute vertex visits/s (validation baseline)",
+        "value": value,
+        "unit": "vertex visits/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1000.0 * wall / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (reference generator, seed 42)",
+        "config": {"workload": f"IrLP on the final state of the {B}-batch stream of a "
+                               f"{N}-vertex, degree-{args.degree} graph, delta 1e-4",
+                   "vertices": N, "active_entries": E, "iterations": its,
+                   "group_lanes": best, "lanes_sweep_loop_ms": sweep},
+        "irlp": {"loop_ms": loop_ms, "validation_call_ms": time_ms,
+                 "per_iteration_us": per_iter_s * 1e6, "accuracy": runs[0]["accuracy"]},
+        "dynlp": {"total_batch_ms": dyn_ms, "final_batch_accuracy": dyn_acc,
+                  "speedup_vs_irlp": time_ms / dyn_ms if dyn_ms > 0 else None,
+                  "speedup_vs_irlp_loop_only": loop_ms / dyn_ms if dyn_ms > 0 else None},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_irlp",
+                     "avg_launch_us": per_iter_s * 1e6,
+                     "algorithmic_bytes_per_launch": byts},
+    }
+    if not args.no_cpu:
+        from oracle.oracle import irlp as oracle_irlp, irlp_final_state
+        rp, ci, vv, st, gt, lab = irlp_final_state(inputs)
+        t1 = time.perf_counter()
+        oracle_irlp(rp, ci, vv, lab, st, gt, 1e-4, 5)
+        five = time.perf_counter() - t1
+        n = int(min(its, max(5, 20.0 / max(five / 5, 1e-9))))
+        t1 = time.perf_counter()
+        oracle_irlp(rp, ci, vv, lab, st, gt, 1e-4, n)
+        dt = time.perf_counter() - t1
+        out["cpu_baseline"] = {"value": N * n / dt, "unit": "vertex visits/s", "cores": 1,
+                               "kind": "port",
+                               "sample": f"first {n} of {its} IrLP iterations on the same "
+                                         f"final state, {dt:.1f} s, CPU oracle "
+                                         "(single thread)"}
+    if args.calibrate_copy_mib > 0:
+        out["copy_calibration"] = copy_calibration(args.calibrate_copy_mib)
+    print(json.dumps(out))
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -323,10 +410,11 @@ def main() -> int:
     ap.add_argument("--local-shape", type=int, default=0,
                     help="work-group shape of the LDS-resident kernel (1-based)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--workload", choices=("c2", "batched", "serving"), default="c2",
+    ap.add_argument("--workload", choices=("c2", "batched", "serving", "irlp"), default="c2",
                     help="c2: one 500,000-vertex graph (default); batched: many graphs, "
                          "one propagation launch per batch index; serving: arrival-driven "
-                         "continuous batching over resident tenant graphs")
+                         "continuous batching over resident tenant graphs; irlp: the "
+                         "full-recompute baseline on the final state of the c2 stream")
     ap.add_argument("--tenants", type=int, default=64, help="tenant graphs (serving)")
     ap.add_argument("--requests", type=int, default=128, help="update batches (serving)")
     ap.add_argument("--max-tenant-nodes", type=float, default=5e6,
@@ -354,6 +442,8 @@ def main() -> int:
         return batched_main(args)
     if args.workload == "serving":
         return serving_main(args)
+    if args.workload == "irlp":
+        return irlp_main(args)
 
     inputs = L.generate(args.nodes, args.degree, args.batches, 42)
     cfg = L.make_config(inputs)

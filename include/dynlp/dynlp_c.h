@@ -6,7 +6,7 @@
  * point returns DLP_OK (0) or a DLP_ERR_* code; dlp_last_error() gives the
  * message of the last failure on the calling thread.
  *
- * Three groups of entry points:
+ * Groups of entry points:
  *   1. Step API -- one entry per reference step launcher, same argument order
  *      and meaning; raw device pointers; host scalars that the reference
  *      returns through `int&` come back through `int*` (these calls
@@ -16,6 +16,11 @@
  *      stream resident in HBM, for replay, tracing and measurement.
  *   4. Input generation -- the reference's synthetic graph and batch
  *      generators (host code), so that callers get identical inputs.
+ *   5. Many independent graphs -- one propagation launch per batch index
+ *      over many resident graphs, and an arrival-driven serving loop.
+ *   6. Device-memory plumbing (allocation and copies) for callers without a
+ *      GPU runtime of their own.
+ *   7. IrLP -- the reference's full-recompute validation baseline.
  *
  * Precision contract: FP32 with IEEE division and no FMA contraction; the
  * propagation sum follows the reference kernel's 256-slot block reduction
@@ -333,7 +338,7 @@ int dlp_inputs_batches(const dlp_inputs *in, dlp_batch_view *batches);
 void dlp_inputs_free(dlp_inputs *in);
 
 /* ====================================================================== */
-/* 5b. Many independent graphs: one propagation launch per batch index    */
+/* 5. Many independent graphs: one propagation launch per batch index     */
 /*    (no counterpart in the reference, which updates one graph at a     */
 /*    time; its per-graph semantics are runDynLP's, dynLP.cu:175-587)     */
 /* ====================================================================== */
@@ -396,7 +401,7 @@ int dlp_multi_serve(dlp_multi *m, const dlp_serve_request *reqs, int numReqs,
                     const dlp_serve_config *cfg, double *completion, int *steps);
 
 /* ====================================================================== */
-/* 5. Device-memory plumbing for callers without a GPU runtime of their   */
+/* 6. Device-memory plumbing for callers without a GPU runtime of their   */
 /*    own (the reference calls cudaMalloc/cudaMemcpy directly)            */
 /* ====================================================================== */
 int dlp_device_malloc(void **ptr, size_t bytes);
@@ -408,7 +413,7 @@ int dlp_copy_to_host(void *dst, const void *src, size_t bytes);
 int dlp_copy_device(void *dst, const void *src, size_t bytes);
 
 /* ====================================================================== */
-/* 6. IrLP: the full-recompute baseline (validation.h, main.cu:120-217)  */
+/* 7. IrLP: the full-recompute baseline (validation.h, main.cu:120-217)  */
 /* ====================================================================== */
 
 /* Replaces launchIrLP (validation.h, validation.cu:144-205).  Device
@@ -451,6 +456,12 @@ int dlp_irlp_ex(const int *rowPtr, const int *colInd, const float *values, float
 int dlp_irlp_validation(const dlp_graph_view *activeGraph, const int *vertexStatus,
                         const int *gtClass, float *labels, int numNodes, float delta,
                         int maxIterations, int *iterationsUsed, int *skipped);
+/* dlp_irlp_validation with a launch shape (nullable) and the stream time of
+ * the iteration loop alone in *loopMs (nullable). */
+int dlp_irlp_validation_ex(const dlp_graph_view *activeGraph, const int *vertexStatus,
+                           const int *gtClass, float *labels, int numNodes, float delta,
+                           int maxIterations, int *iterationsUsed, int *skipped,
+                           const dlp_irlp_tuning *tuning, float *loopMs);
 
 typedef struct dlp_irlp_result {
   int iterations;      /* iterationsUsed                                   */
@@ -458,6 +469,7 @@ typedef struct dlp_irlp_result {
   int activeEdges;     /* entries of the final active CSR                  */
   float timeMs;        /* wall time of the validation call (upload, run,
                           download), the span main.cu:177-197 times        */
+  float loopMs;        /* stream time of the iteration loop alone         */
   float accuracy;      /* over non-deleted, non-ground-truth vertices       */
 } dlp_irlp_result;
 
@@ -466,10 +478,11 @@ typedef struct dlp_irlp_result {
  * ground truth its label), builds the active CSR (entries of the full graph
  * between inserted, non-deleted vertices), runs dlp_irlp_validation with
  * cfg->delta / cfg->maxIterations and scores the result against trueLabels.
- * irlpLabels: numNodes floats. */
+ * irlpLabels: numNodes floats; tuning nullable. */
 int dlp_irlp_baseline(const dlp_config *cfg, const dlp_graph_view *fullGraph,
                       const float *trueLabels, const dlp_batch_view *batches,
-                      float *irlpLabels, dlp_irlp_result *out);
+                      const dlp_irlp_tuning *tuning, float *irlpLabels,
+                      dlp_irlp_result *out);
 
 #ifdef __cplusplus
 }

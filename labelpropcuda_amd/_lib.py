@@ -170,6 +170,7 @@ class IrlpResult(C.Structure):
         ("skipped", C.c_int),
         ("activeEdges", C.c_int),
         ("timeMs", C.c_float),
+        ("loopMs", C.c_float),
         ("accuracy", C.c_float),
     ]
 
@@ -248,8 +249,12 @@ SIGNATURES = {
                               C.POINTER(IrlpTuning), f32p]),
     "dlp_irlp_validation": (C.c_int, [C.POINTER(GraphView), i32p, i32p, f32p, C.c_int, C.c_float,
                                       C.c_int, i32p, i32p]),
+    "dlp_irlp_validation_ex": (C.c_int, [C.POINTER(GraphView), i32p, i32p, f32p, C.c_int,
+                                         C.c_float, C.c_int, i32p, i32p, C.POINTER(IrlpTuning),
+                                         f32p]),
     "dlp_irlp_baseline": (C.c_int, [C.POINTER(Config), C.POINTER(GraphView), f32p,
-                                    C.POINTER(BatchView), f32p, C.POINTER(IrlpResult)]),
+                                    C.POINTER(BatchView), C.POINTER(IrlpTuning), f32p,
+                                    C.POINTER(IrlpResult)]),
     "dlp_device_malloc": (C.c_int, [C.POINTER(_VP), C.c_size_t]),
     "dlp_device_free": (C.c_int, [_VP]),
     "dlp_copy_to_device": (C.c_int, [_VP, _VP, C.c_size_t]),

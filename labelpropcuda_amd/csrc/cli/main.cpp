@@ -51,7 +51,8 @@ int main(int argc, char *argv[]) {
     dlp_graph_view gv = graphView(fullGraph);
     dlp_config c = toCConfig(config);
     irlpLabels.assign(config.totalNodes, 0.5f);
-    DLP_CHECK(dlp_irlp_baseline(&c, &gv, trueLabels.data(), &bv, irlpLabels.data(), &irlp));
+    DLP_CHECK(dlp_irlp_baseline(&c, &gv, trueLabels.data(), &bv, nullptr, irlpLabels.data(),
+                                &irlp));
     if (irlp.skipped) std::cout << "Warning: IrLP validation skipped (empty graph).\n";
     std::cout << "IrLP baseline: " << irlp.iterations << " iterations, " << irlp.timeMs
               << " ms, accuracy " << std::fixed << std::setprecision(2)

@@ -448,15 +448,18 @@ extern "C" int dlp_irlp(const int *rowPtr, const int *colInd, const float *value
                      maxIterations, iterationsUsed, stream, nullptr, nullptr);
 }
 
-extern "C" int dlp_irlp_validation(const dlp_graph_view *activeGraph, const int *vertexStatus,
-                                   const int *gtClass, float *labels, int numNodes, float delta,
-                                   int maxIterations, int *iterationsUsed, int *skipped) {
+extern "C" int dlp_irlp_validation_ex(const dlp_graph_view *activeGraph,
+                                      const int *vertexStatus, const int *gtClass, float *labels,
+                                      int numNodes, float delta, int maxIterations,
+                                      int *iterationsUsed, int *skipped,
+                                      const dlp_irlp_tuning *tuning, float *loopMs) {
   if (!activeGraph || !iterationsUsed) {
     dlp_set_error_("dlp_irlp_validation: NULL argument");
     return DLP_ERR_INVALID;
   }
   *iterationsUsed = 0;
   if (skipped) *skipped = 0;
+  if (loopMs) *loopMs = 0.0f;
   if (numNodes <= 0 || activeGraph->numEdges <= 0) { /* validation.cu:216-219 */
     if (skipped) *skipped = 1;
     return DLP_OK;
@@ -484,10 +487,10 @@ extern "C" int dlp_irlp_validation(const dlp_graph_view *activeGraph, const int 
     rc = DLP_ERR_HIP;
   }
   if (rc == DLP_OK)
-    rc = dlp_irlp(static_cast<const int *>(buf[0]), static_cast<const int *>(buf[1]),
-                  static_cast<const float *>(buf[2]), static_cast<float *>(buf[3]),
-                  static_cast<const int *>(buf[4]), static_cast<const int *>(buf[5]), numNodes,
-                  delta, maxIterations, iterationsUsed, s);
+    rc = dlp_irlp_ex(static_cast<const int *>(buf[0]), static_cast<const int *>(buf[1]),
+                     static_cast<const float *>(buf[2]), static_cast<float *>(buf[3]),
+                     static_cast<const int *>(buf[4]), static_cast<const int *>(buf[5]),
+                     numNodes, delta, maxIterations, iterationsUsed, s, tuning, loopMs);
   if (rc == DLP_OK) {
     e = hipMemcpyAsync(labels, buf[3], bytes[3], hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
@@ -501,4 +504,11 @@ extern "C" int dlp_irlp_validation(const dlp_graph_view *activeGraph, const int 
     if (buf[i]) (void)hipFree(buf[i]);
   if (s) (void)hipStreamDestroy(s);
   return rc;
+}
+
+extern "C" int dlp_irlp_validation(const dlp_graph_view *activeGraph, const int *vertexStatus,
+                                   const int *gtClass, float *labels, int numNodes, float delta,
+                                   int maxIterations, int *iterationsUsed, int *skipped) {
+  return dlp_irlp_validation_ex(activeGraph, vertexStatus, gtClass, labels, numNodes, delta,
+                                maxIterations, iterationsUsed, skipped, nullptr, nullptr);
 }

@@ -35,7 +35,8 @@ bool batchesValid(const dlp_batch_view *b, int N) {
 
 extern "C" int dlp_irlp_baseline(const dlp_config *cfg, const dlp_graph_view *fullGraph,
                                  const float *trueLabels, const dlp_batch_view *batches,
-                                 float *irlpLabels, dlp_irlp_result *out) {
+                                 const dlp_irlp_tuning *tuning, float *irlpLabels,
+                                 dlp_irlp_result *out) {
   if (!cfg || !fullGraph || !trueLabels || !batches || !irlpLabels || !out) {
     dlp_set_error_("dlp_irlp_baseline: NULL argument");
     return DLP_ERR_INVALID;
@@ -94,8 +95,9 @@ extern "C" int dlp_irlp_baseline(const dlp_config *cfg, const dlp_graph_view *fu
     out->activeEdges = active.numEdges;
 
     const auto t0 = std::chrono::steady_clock::now();
-    int rc = dlp_irlp_validation(&active, status.data(), gtClass.data(), irlpLabels, N, cfg->delta,
-                                 cfg->maxIterations, &out->iterations, &out->skipped);
+    int rc = dlp_irlp_validation_ex(&active, status.data(), gtClass.data(), irlpLabels, N,
+                                    cfg->delta, cfg->maxIterations, &out->iterations,
+                                    &out->skipped, tuning, &out->loopMs);
     const auto t1 = std::chrono::steady_clock::now();
     if (rc != DLP_OK) return rc;
     out->timeMs = std::chrono::duration<float, std::milli>(t1 - t0).count();

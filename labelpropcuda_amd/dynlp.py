@@ -236,16 +236,17 @@ def irlp_validation(rowPtr, colInd, values, status, gtClass, labels, delta: floa
     return out, it.value, bool(skipped.value)
 
 
-def irlp_baseline(inputs: Inputs, cfg: _lib.Config):
+def irlp_baseline(inputs: Inputs, cfg: _lib.Config, tuning: _lib.IrlpTuning | None = None):
     """The --validate step (main.cu:124-217): IrLP on the final state of the
     batch stream.  Returns (irlpLabels, {iterations, skipped, activeEdges,
-    timeMs, accuracy})."""
+    timeMs, loopMs, accuracy})."""
     lib = load()
     labels = np.zeros(inputs.numNodes, np.float32)
     res = _lib.IrlpResult()
     g, bv = inputs.graph_view(), inputs.batch_view()
     check(lib.dlp_irlp_baseline(C.byref(cfg), C.byref(g), _ptr(inputs.trueLabels, C.c_float),
-                                C.byref(bv), _ptr(labels, C.c_float), C.byref(res)))
+                                C.byref(bv), C.byref(tuning) if tuning is not None else None,
+                                _ptr(labels, C.c_float), C.byref(res)))
     return labels, {f: getattr(res, f) for f, _ in res._fields_}
 
 

@@ -130,3 +130,40 @@ def test_cli_help_and_flag_errors():
     assert r.returncode == 1 and "--delta must be positive" in r.stdout
     r = subprocess.run([_lib.CLI_PATH, "--bogus"], capture_output=True, text=True)
     assert r.returncode == 1 and "Unknown argument" in r.stdout
+
+
+def header_structs():
+    """typedef struct dlp_x {...} dlp_x; names in the C header."""
+    text = re.sub(r"/\*.*?\*/", "", open(HEADER).read(), flags=re.S)
+    return re.findall(r"typedef struct (dlp_[a-z_]+) \{", text)
+
+
+def test_ctypes_structs_match_the_header_layout(tmp_path):
+    """Every ctypes mirror of a header struct has the C field names, offsets
+    and size (a field added or dropped on one side fails here)."""
+    pairs = {"dlp_config": _lib.Config, "dlp_graph_view": _lib.GraphView,
+             "dlp_batch_view": _lib.BatchView, "dlp_batch_stats": _lib.BatchStats,
+             "dlp_tuning": _lib.Tuning, "dlp_trace": _lib.Trace,
+             "dlp_multi_tuning": _lib.MultiTuning, "dlp_serve_request": _lib.ServeRequest,
+             "dlp_serve_config": _lib.ServeConfig, "dlp_prop_profile": _lib.PropProfile,
+             "dlp_irlp_tuning": _lib.IrlpTuning, "dlp_irlp_result": _lib.IrlpResult}
+    assert set(header_structs()) == set(pairs), "a header struct has no ctypes mirror"
+    lines = ['#include <stddef.h>', '#include <stdio.h>', '#include "dynlp/dynlp_c.h"',
+             'int main(void) {']
+    for cname, cls in pairs.items():
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for f, _ in cls._fields_:
+            lines.append(f'printf("{cname} {f} %zu\\n", offsetof({cname}, {f}));')
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)],
+                   check=True, capture_output=True)
+    got = dict(line.rsplit(" ", 1) for line in
+               subprocess.run([str(exe)], check=True, capture_output=True,
+                              text=True).stdout.splitlines())
+    for cname, cls in pairs.items():
+        assert int(got[f"{cname} size"]) == C.sizeof(cls), cname
+        for f, _ in cls._fields_:
+            assert int(got[f"{cname} {f}"]) == getattr(cls, f).offset, f"{cname}.{f}"

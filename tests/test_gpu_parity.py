@@ -98,6 +98,7 @@ def test_c2_matches_oracle_every_batch(gpu_available):
             assert_labels_equal(gpu, orc.labels(), f"C2 batch {b}")
             st, ost = ctx.stats(b), orc.stats()
             assert (st["iterations"], st["updates"]) == (ost["iterations"], ost["updates"])
+            assert st["accuracy"] == ost["accuracy"], f"C2 batch {b}: accuracy"
             err = np.abs(gpu.astype(np.float64) - o64.labels())
             print(f"C2 batch {b}: iterations {st['iterations']} updates {st['updates']} "
                   f"|F32-F64| max {err.max():.3e} mean {err.mean():.3e}")
