@@ -307,6 +307,91 @@ def serving_main(args) -> int:
     return 0
 
 
+def stream_main(args) -> int:
+    """One pass over a large stream (C3: 50,000,000 vertices, degree 7, 20
+    batches), batch by batch with progress lines, for per-batch latency and
+    per-engine propagation time; a second pass collects the work counters.
+    Per-kernel GB/s come from rocprofv3 runs of this mode."""
+    import labelpropcuda_amd as L
+    t0 = time.perf_counter()
+
+    def progress(msg):
+        print(f"[bench stream {time.perf_counter() - t0:7.1f} s] {msg}", file=sys.stderr,
+              flush=True)
+
+    cache = args.inputs_cache
+    if cache and os.path.exists(cache):
+        inputs = L.Inputs.load(cache)
+        progress(f"loaded {cache}")
+    else:
+        inputs = L.generate(args.nodes, args.degree, args.batches, 42)
+        if cache:
+            inputs.save(cache)
+    N, B = inputs.numNodes, inputs.numBatches
+    assert (N, B) == (args.nodes, args.batches), "inputs cache holds another workload"
+    progress(f"generated {N} vertices, {inputs.numEdges} entries, {B} batches")
+    cfg = L.make_config(inputs)
+    ctx = L.Context(inputs, cfg)
+    progress(f"context resident, {ctx.device_bytes() / 2**30:.2f} GiB")
+    ctx.set_profiling(1)
+    ctx.reset()
+    ctx.synchronize()
+    stats = []
+    tw = time.perf_counter()
+    for b in range(B):
+        ctx.run_batch(b)
+        ctx.synchronize()
+        st = ctx.stats(b)
+        stats.append(st)
+        progress(f"batch {b}: {st['iterations']} iterations, {st['updates']} updates, "
+                 f"{st['timeMs']:.1f} ms")
+    wall = time.perf_counter() - tw
+    timed = ctx.prop_profile()
+    updates = int(sum(s["updates"] for s in stats))
+    prof = None
+    if not args.no_work_counters:
+        ctx.set_profiling(2)
+        ctx.run_all()
+        ctx.synchronize()
+        prof = ctx.prop_profile()
+        for key in ("launches", "kernelMs", "localLaunches", "localKernelMs"):
+            prof[key] = timed[key]
+        progress("work counters collected")
+    ctx.set_profiling(0)
+    ctx.close()
+    lat = [s["timeMs"] for s in stats]
+    out = {
+        "metric": "affected-vertex updates/s (plain 1-GPU loop)",
+        "value": updates / wall,
+        "unit": "updates/s",
+        "n_gpus": 1,
+        "steps": 1,
+        "warmup": 0,
+        "ms_per_step": 1000.0 * wall,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (reference generator, seed 42)",
+        "config": {"workload": f"DynLP plain loop, {N} vertices, avg degree {args.degree}, "
+                               f"{B} batches (90/1/9), tau=mean weight, delta 1e-4",
+                   "vertices": N, "entries": inputs.numEdges, "batches": B,
+                   "updates_per_step": updates},
+        "batch_latency_ms": {"p50": float(np.percentile(lat, 50)),
+                             "p99": float(np.percentile(lat, 99))},
+        "batches": [{k: s[k] for k in ("iterations", "updates", "frontier0", "numSubEdges",
+                                       "numComponents", "timeMs", "accuracy")} for s in stats],
+        "propagation_time_ms": {"k_propagate": timed["kernelMs"],
+                                "k_propagate_local": timed["localKernelMs"],
+                                "launches": timed["launches"],
+                                "local_launches": timed["localLaunches"]},
+    }
+    if prof is not None:
+        out["propagation_kernels"] = engine_rooflines(prof, updates)
+    print(json.dumps(out))
+    return 0
+
+
 def irlp_main(args) -> int:
     """IrLP, the reference's full-recompute baseline (--validate), on the
     final state of the same 10-batch stream, beside the DynLP stream time it
@@ -410,11 +495,19 @@ def main() -> int:
     ap.add_argument("--local-shape", type=int, default=0,
                     help="work-group shape of the LDS-resident kernel (1-based)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--workload", choices=("c2", "batched", "serving", "irlp"), default="c2",
+    ap.add_argument("--workload", choices=("c2", "batched", "serving", "irlp", "stream"),
+                    default="c2",
                     help="c2: one 500,000-vertex graph (default); batched: many graphs, "
                          "one propagation launch per batch index; serving: arrival-driven "
                          "continuous batching over resident tenant graphs; irlp: the "
-                         "full-recompute baseline on the final state of the c2 stream")
+                         "full-recompute baseline on the final state of the c2 stream; "
+                         "stream: one pass over --nodes/--degree/--batches batch by batch "
+                         "(C3: --nodes 50000000 --batches 20)")
+    ap.add_argument("--inputs-cache", default="",
+                    help="stream: .npz of the generated inputs, written on the first run "
+                         "and loaded by later ones")
+    ap.add_argument("--no-work-counters", action="store_true",
+                    help="stream: skip the second pass that collects work counters")
     ap.add_argument("--tenants", type=int, default=64, help="tenant graphs (serving)")
     ap.add_argument("--requests", type=int, default=128, help="update batches (serving)")
     ap.add_argument("--max-tenant-nodes", type=float, default=5e6,
@@ -444,6 +537,8 @@ def main() -> int:
         return serving_main(args)
     if args.workload == "irlp":
         return irlp_main(args)
+    if args.workload == "stream":
+        return stream_main(args)
 
     inputs = L.generate(args.nodes, args.degree, args.batches, 42)
     cfg = L.make_config(inputs)

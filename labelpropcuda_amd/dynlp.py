@@ -66,6 +66,18 @@ class Inputs:
             "eW": self.eW[s(self.eOff[b], self.eOff[b + 1])],
         }
 
+    _ARRAYS = ("rowPtr", "colInd", "values", "trueLabels", "insOff", "ins", "gtOff", "gtIds",
+               "gtLab", "delOff", "dels", "eOff", "eSrc", "eDst", "eW")
+
+    def save(self, path: str) -> None:
+        """Binary snapshot of the generator's output (uncompressed .npz)."""
+        np.savez(path, **{k: getattr(self, k) for k in self._ARRAYS})
+
+    @classmethod
+    def load(cls, path: str) -> "Inputs":
+        with np.load(path, allow_pickle=False) as z:
+            return cls(**{k: np.ascontiguousarray(z[k]) for k in cls._ARRAYS})
+
     def graph_view(self) -> _lib.GraphView:
         g = _lib.GraphView()
         g.numNodes = self.numNodes
