@@ -100,22 +100,39 @@ def copy_calibration(mib: int) -> dict:
     return {"bytes": n, "wall_s": dt}
 
 
-def cpu_baseline(inputs, budget_s: float = 40.0):
-    """The CPU oracle (single thread) on the same stream, batch by batch,
-    stopping once the time budget is spent; returns updates/s over the
-    batches run."""
-    from oracle.oracle import Oracle
-    orc = Oracle(inputs)
-    done, upd, t = 0, 0, 0.0
-    for b in range(inputs.numBatches):
-        t0 = time.perf_counter()
-        orc.run_batch(b)
-        t += time.perf_counter() - t0
-        upd += orc.stats()["updates"]
-        done += 1
-        if t > budget_s:
-            break
+def cpu_baseline(inputs, budget_s: float = 20.0, threads: int = 1):
+    """The CPU oracle on the same stream, batch by batch, stopping once the
+    time budget is spent; returns updates/s over the batches run.  threads
+    > 1 runs its frontier-parallel propagation (same results)."""
+    from oracle.oracle import Oracle, set_threads
+    set_threads(threads)
+    try:
+        orc = Oracle(inputs)
+        done, upd, t = 0, 0, 0.0
+        for b in range(inputs.numBatches):
+            t0 = time.perf_counter()
+            orc.run_batch(b)
+            t += time.perf_counter() - t0
+            upd += orc.stats()["updates"]
+            done += 1
+            if t > budget_s:
+                break
+    finally:
+        set_threads(1)
     return upd / t, done, t
+
+
+def cpu_baselines(inputs) -> dict:
+    """The oracle on all host cores (at most 16, the box's share) as
+    `cpu_baseline`, and single-threaded beside it."""
+    cores = max(1, min(16, os.cpu_count() or 1))
+    out = {}
+    for key, th in (("cpu_baseline", cores), ("cpu_baseline_1thread", 1)):
+        v, done, t = cpu_baseline(inputs, threads=th)
+        out[key] = {"value": v, "unit": "updates/s", "cores": th, "kind": "port",
+                    "sample": f"first {done} of {inputs.numBatches} batches of the same stream, "
+                              f"{t:.1f} s, CPU oracle ({'frontier-parallel, OpenMP' if th > 1 else 'single thread'})"}
+    return out
 
 
 def batched_main(args) -> int:
@@ -153,20 +170,28 @@ def batched_main(args) -> int:
 
     m = L.Multi(graphs)
     progress("multi-graph runner created")
-    # autotuner: work-group shape x LDS-pass threshold, one step each
+    # autotuner: work-group shape x hand-back frontier, one step each; a
+    # setting whose capturing step is already 1.5x the best so far is dropped
     trials = []
     for shape in (3, 1):
-        for pct in (100,):
-            m.set_tuning(localShape=shape, thresholdPercent=pct, useGraphs=1)
+        for heavy in args.heavy_sweep:
+            m.set_tuning(localShape=shape, heavyFrontier=heavy, useGraphs=1)
+            t1 = time.perf_counter()
             m.run_all()  # captures the batch graphs
             m.synchronize()
+            first = time.perf_counter() - t1
+            progress(f"autotuner: shape {shape} hand-back {heavy}: capturing step "
+                     f"{1000.0 * first:.1f} ms")
+            if trials and first > 1.5 * min(trials)[0]:
+                continue
             t1 = time.perf_counter()
             m.run_all()
             m.synchronize()
-            trials.append((time.perf_counter() - t1, shape, pct))
-            progress(f"autotuner: shape {shape} threshold {pct}%: {1000.0 * trials[-1][0]:.1f} ms")
+            trials.append((time.perf_counter() - t1, shape, heavy))
+            progress(f"autotuner: shape {shape} hand-back {heavy}: "
+                     f"{1000.0 * trials[-1][0]:.1f} ms")
     best = min(trials)
-    m.set_tuning(localShape=best[1], thresholdPercent=best[2], useGraphs=1)
+    m.set_tuning(localShape=best[1], heavyFrontier=best[2], useGraphs=1)
     for _ in range(max(args.warmup, 1)):
         m.run_all()
     m.synchronize()
@@ -213,8 +238,8 @@ def batched_main(args) -> int:
                                             "p99": float(np.percentile(plain_lat, 99))}},
         "speedup_vs_plain": value / plain_value,
         "labels_identical_to_plain": bool(same),
-        "autotuner": [{"shape": s_, "thresholdPercent": p_, "ms": 1000.0 * t_}
-                      for t_, s_, p_ in trials],
+        "autotuner": [{"shape": s_, "heavyFrontier": h_, "ms": 1000.0 * t_}
+                      for t_, s_, h_ in trials],
         "tuning": tuning,
     }
     print(json.dumps(out))
@@ -246,6 +271,8 @@ def serving_main(args) -> int:
             progress(f"generated {t + 1} tenants")
     progress(f"generated {T} tenants, {int(sizesN.sum())} vertices, {R} requests")
     m = L.Multi(tenants)
+    if args.heavy_frontier:
+        m.set_tuning(heavyFrontier=args.heavy_frontier)
     progress("tenants resident")
 
     def run(arrivals, plain):
@@ -279,6 +306,7 @@ def serving_main(args) -> int:
         sweep.append(row)
         progress(f"rate {f} x capacity: batched p99 {row['batched']['p99_ms']:.1f} ms, "
                  f"plain p99 {row['plain']['p99_ms']:.1f} ms")
+    tuning = m.tuning()
     m.close()
     value = sat_batched["updates_per_s"]
     out = {
@@ -299,6 +327,7 @@ def serving_main(args) -> int:
                                "log-uniform 100-100000 vertices (90/1/9)",
                    "slots": args.slots, "hbm_budget_bytes": args.hbm_budget,
                    "updates": sat_batched["updates"]},
+        "tuning": tuning,
         "saturated": {"batched": sat_batched, "plain": sat_plain,
                       "speedup_vs_plain": value / sat_plain["updates_per_s"]},
         "rate_sweep": sweep,
@@ -332,7 +361,9 @@ def stream_main(args) -> int:
     progress(f"generated {N} vertices, {inputs.numEdges} entries, {B} batches")
     cfg = L.make_config(inputs)
     ctx = L.Context(inputs, cfg)
-    progress(f"context resident, {ctx.device_bytes() / 2**30:.2f} GiB")
+    ctx.set_tuning(args.lanes, args.blocks, args.chunk, args.local_threshold,
+                   localShape=args.local_shape)
+    progress(f"context resident, {ctx.device_bytes() / 2**30:.2f} GiB, tuning {ctx.tuning()}")
     ctx.set_profiling(1)
     ctx.reset()
     ctx.synchronize()
@@ -358,6 +389,7 @@ def stream_main(args) -> int:
             prof[key] = timed[key]
         progress("work counters collected")
     ctx.set_profiling(0)
+    tuning = ctx.tuning()
     ctx.close()
     lat = [s["timeMs"] for s in stats]
     out = {
@@ -379,6 +411,7 @@ def stream_main(args) -> int:
                    "updates_per_step": updates},
         "batch_latency_ms": {"p50": float(np.percentile(lat, 50)),
                              "p99": float(np.percentile(lat, 99))},
+        "tuning": tuning,
         "batches": [{k: s[k] for k in ("iterations", "updates", "frontier0", "numSubEdges",
                                        "numComponents", "timeMs", "accuracy")} for s in stats],
         "propagation_time_ms": {"k_propagate": timed["kernelMs"],
@@ -388,6 +421,8 @@ def stream_main(args) -> int:
     }
     if prof is not None:
         out["propagation_kernels"] = engine_rooflines(prof, updates)
+    if args.calibrate_copy_mib > 0:
+        out["copy_calibration"] = copy_calibration(args.calibrate_copy_mib)
     print(json.dumps(out))
     return 0
 
@@ -409,7 +444,7 @@ def irlp_main(args) -> int:
 
     # launch shape: lanes per vertex with the shortest loop
     sweep = {}
-    for g in (4, 8, 16, 32):
+    for g in (1, 4, 8, 16):
         _, r = L.dynlp.irlp_baseline(inputs, cfg, L.dynlp.irlp_tuning(groupLanes=g))
         sweep[g] = r["loopMs"]
     best = min(sweep, key=sweep.get)
@@ -428,7 +463,7 @@ def irlp_main(args) -> int:
     time_ms = float(np.mean([r["timeMs"] for r in runs]))
     E = runs[0]["activeEdges"]
     per_iter_s = loop_ms / 1000.0 / its
-    byts = 20 * N + 16 * E  # per sweep: vertex 20 B, stored entry 16 B
+    byts = 20 * N + 12 * E  # per sweep: vertex 20 B, stored entry 12 B
     achieved = byts / per_iter_s / 1e9
     value = N * its / (loop_ms / 1000.0)
     out = {
@@ -518,6 +553,11 @@ def main() -> int:
     ap.add_argument("--rates", type=float, nargs="*", default=[0.5, 1.0, 2.0, 4.0],
                     help="arrival rates as multiples of the plain path's capacity (serving)")
     ap.add_argument("--graphs", type=int, default=256, help="graphs of the batched workload")
+    ap.add_argument("--heavy-sweep", type=int, nargs="*", default=[-1, 65536],
+                    help="batched: hand-back frontier sizes the autotuner tries")
+    ap.add_argument("--heavy-frontier", type=int, default=0,
+                    help="serving: frontier size at which a graph leaves the shared launch "
+                         "(0 = library default, -1 = never)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="after the run, copy this many MiB device to device with the "
                          "library's copy kernel (reference point for HBM counters)")
@@ -615,10 +655,7 @@ def main() -> int:
         "tuning": tuning,
     }
     if not args.no_cpu:
-        v, done, t = cpu_baseline(inputs)
-        out["cpu_baseline"] = {"value": v, "unit": "updates/s", "cores": 1, "kind": "port",
-                               "sample": f"first {done} of {inputs.numBatches} batches of the "
-                                         f"same stream, {t:.1f} s, CPU oracle (single thread)"}
+        out.update(cpu_baselines(inputs))
     if args.calibrate_copy_mib > 0:
         out["copy_calibration"] = copy_calibration(args.calibrate_copy_mib)
     print(json.dumps(out))

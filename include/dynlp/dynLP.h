@@ -119,6 +119,20 @@ bool generateBatches(const CSRGraph &graph,
                      const DynLPConfig &config,
                      std::vector<BatchData> &batches);
 
+/* The generator's CSR text dump (<prefix>RowPtr.txt, ColInd.txt, Values.txt,
+ * TrueLabels.txt; generateSparseGraph.cu:217-274). */
+bool writeCSRGraph(const std::string &prefix, const CSRGraph &graph,
+                   const std::vector<float> &trueLabels);
+
+/* ---- readers of the written files (no counterpart in the reference) ------ */
+/* the CSR dump above (weights as written, 6 decimals) */
+bool readCSRGraph(const std::string &prefix, CSRGraph &graph, std::vector<float> &trueLabels);
+/* dynlp_labels.txt / irlp_labels.txt */
+bool readLabelsFile(const std::string &path, std::vector<float> &labels);
+/* results.csv */
+bool readResultsCSV(const std::string &path, std::vector<float> &batchTimes,
+                    std::vector<float> &batchAccuracy);
+
 /* ---- CLI / file utilities ------------------------------------------------ */
 bool parseArguments(int argc, char *argv[], DynLPConfig &config);
 void printConfig(const DynLPConfig &config);

@@ -337,6 +337,18 @@ int dlp_inputs_graph(const dlp_inputs *in, dlp_graph_view *graph,
 int dlp_inputs_batches(const dlp_inputs *in, dlp_batch_view *batches);
 void dlp_inputs_free(dlp_inputs *in);
 
+/* The generator's CSR text dump (generateSparseGraph.cu:217-274):
+ * <prefix>RowPtr.txt, ColInd.txt, Values.txt (6 decimals) and
+ * TrueLabels.txt ("id label"); directories of the prefix are created. */
+int dlp_write_graph_files(const dlp_graph_view *graph, const float *trueLabels,
+                          const char *prefix);
+/* A graph read back from such files (weights as written) with the
+ * reference's batch generator run over it (generateBatches.cu:44-223,
+ * seed + 12345), so a stored graph can be replayed. */
+int dlp_inputs_from_graph_files(dlp_inputs **out, const char *prefix, int numBatches,
+                                unsigned int seed, float groundTruthFraction,
+                                float deletedFraction);
+
 /* ====================================================================== */
 /* 5. Many independent graphs: one propagation launch per batch index     */
 /*    (no counterpart in the reference, which updates one graph at a     */
@@ -352,6 +364,12 @@ typedef struct dlp_multi_tuning {
   int backoff;          /* HBM iterations after an LDS overflow (8)      */
   int useGraphs;        /* 1 = capture each batch in a HIP graph and
                            replay it (default), 2 = enqueue every time   */
+  int heavyFrontier;    /* a graph whose frontier grows beyond this many
+                           vertices leaves the shared launch and finishes
+                           its propagation on the whole GPU, after the
+                           launch (default -1 = never): lower latency for
+                           the other graphs of a step, but the handed-back
+                           graphs run one after another                  */
 } dlp_multi_tuning;
 
 /* One context per graph (arrays of numGraphs configs, graphs, true-label
@@ -432,7 +450,9 @@ int dlp_irlp(const int *rowPtr, const int *colInd, const float *values, float *l
 
 /* Launch shape of the IrLP sweep (no counterpart in the reference). */
 typedef struct dlp_irlp_tuning {
-  int groupLanes; /* lanes per vertex: 4, 8, 16, 32 or 64                 */
+  int groupLanes; /* lanes per vertex: 1 (default: one lane per row of up
+                     to 16 entries, a wave per longer row), 4, 8, 16, 32
+                     or 64                                                 */
   int blocks;     /* 256-thread blocks per sweep; <= 0: one vertex group
                      per lane group, at most 8192 blocks                   */
   int firstChunk; /* iterations queued before the first convergence check;

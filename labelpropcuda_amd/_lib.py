@@ -128,6 +128,7 @@ class MultiTuning(C.Structure):
         ("thresholdPercent", C.c_int),
         ("backoff", C.c_int),
         ("useGraphs", C.c_int),
+        ("heavyFrontier", C.c_int),
     ]
 
 
@@ -243,6 +244,9 @@ SIGNATURES = {
                                              C.POINTER(C.c_int), C.c_int, C.c_uint, C.c_float,
                                              C.c_float]),
     "dlp_copy_device": (C.c_int, [_VP, _VP, C.c_size_t]),
+    "dlp_write_graph_files": (C.c_int, [C.POINTER(GraphView), f32p, C.c_char_p]),
+    "dlp_inputs_from_graph_files": (C.c_int, [C.POINTER(_VP), C.c_char_p, C.c_int, C.c_uint,
+                                              C.c_float, C.c_float]),
     "dlp_irlp": (C.c_int, [_VP, _VP, _VP, _VP, _VP, _VP, C.c_int, C.c_float, C.c_int, i32p, _VP]),
     "dlp_irlp_tuning_default": (None, [C.POINTER(IrlpTuning)]),
     "dlp_irlp_ex": (C.c_int, [_VP, _VP, _VP, _VP, _VP, _VP, C.c_int, C.c_float, C.c_int, i32p, _VP,

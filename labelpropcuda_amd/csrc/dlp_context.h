@@ -91,6 +91,9 @@ DBatch dlp_ctx_make_batch(const dlp_context *c, int b);
 int dlp_ctx_batch_pre(dlp_context *c, int b, const DBatch &bt, int *markTag);
 /* the arguments of the context's propagation kernels */
 PropArgs dlp_ctx_prop_args(const dlp_context *c);
+/* the context's propagation loop (its tuning, engines and timing) from
+ * iteration startK, the queue and counters of startK being on the device */
+int dlp_ctx_run_propagation(dlp_context *c, int startK);
 /* GT enforcement, accuracy, batch record; marks the batch as run */
 int dlp_ctx_batch_post(dlp_context *c, int b, const DBatch &bt);
 /* the host bookkeeping batch b's pre and post phases perform (mark tag

@@ -87,12 +87,17 @@ struct IrlpArgs {
   float *lab0;        /* the caller's labels: read by even iterations        */
   float *lab1;        /* second buffer, numNodes floats                       */
   int *flags;         /* kIrlpMaxChunk + 1 words                              */
+  int *colMasked;     /* nullable, numEntries ints: filled by dlp_run_irlp
+                         with colInd, every entry that points at a DELETED
+                         vertex set to -1, so sweeps read no neighbour status */
+  long long numEntries; /* rowPtr[numNodes] (with colMasked)                  */
   int numNodes;
   float delta;
 };
 constexpr int kIrlpMaxChunk = 256;
 struct IrlpLoopConfig {
-  int groupLanes = 8;  /* lanes per vertex, 4 .. 64                           */
+  int groupLanes = 1;  /* lanes per vertex: 1 (one lane per row up to 16
+                          entries, a wave per longer row) or 4 .. 64         */
   int blocks = 0;      /* <= 0: enough 256-thread blocks for one vertex group
                           each, at most 8192                                  */
   int firstChunk = 8;  /* iterations queued before the first host check; each
@@ -114,6 +119,10 @@ struct GraphProp {
   int maxIter;
   int threshold;    /* frontier size at or below which LDS passes run */
   int backoff;      /* HBM iterations after an LDS overflow           */
+  int heavy;        /* > 0: a frontier larger than this hands the graph
+                       back to the host at that iteration              */
+  int *exitIter;    /* device int: -1 = propagation finished, else the
+                       iteration to resume at with the full-GPU loop   */
 };
 
 /* Runs the propagation of every graph in `graphs` (device array, count
@@ -150,7 +159,7 @@ struct PropLoopConfig {
  * counter read-backs; `timing` may be null. */
 int dlp_run_propagation(const PropArgs &a, const LocalArgs *l, int maxIter,
                         const PropLoopConfig &cfg, DevCounters *pin, hipStream_t s,
-                        PropTiming *timing);
+                        PropTiming *timing, int startK = 0);
 
 /* ------------------------------------------------------------------------ */
 /* batch pipeline (dlp_batch.hip)                                           */

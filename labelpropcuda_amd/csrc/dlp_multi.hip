@@ -5,15 +5,24 @@
  * index b runs, for every graph that has a batch b, the pipeline up to the
  * first frontier, then ONE propagation launch for all of them (one
  * work-group per graph, k_propagate_multi), then the per-graph batch end.
- * The propagation needs no host decision, so the whole batch -- pipelines,
- * the propagation launch and the batch ends -- is captured once into a HIP
- * graph and replayed on later steps; the state reset of all graphs is a
- * graph of its own.  The reference updates one graph at a time with a host
- * round trip per propagation iteration (dynLP.cu:175-587,
- * iterativePropagation.cu:274-327).
+ * The pipelines and the propagation launch need no host decision, so they
+ * are captured once into a HIP graph per batch index and replayed on later
+ * steps, and so are the batch ends; the state reset of all graphs is a graph
+ * of its own.
+ *
+ * One work-group is slow on a large frontier, so a graph whose frontier
+ * grows beyond `heavy` vertices is handed back: its work-group stops at the
+ * start of that iteration, and after the launch the host continues that
+ * graph's propagation with the single-graph loop (multi-block launches over
+ * the whole GPU, the LDS-resident kernel once the frontier is small again)
+ * before the batch ends run.  The result is the same either way.
+ *
+ * The reference updates one graph at a time with a host round trip per
+ * propagation iteration (dynLP.cu:175-587, iterativePropagation.cu:274-327).
  */
 #include <algorithm>
 #include <chrono>
+#include <functional>
 #include <thread>
 #include <vector>
 
@@ -27,10 +36,14 @@ struct dlp_multi {
   int shape = 2;        /* LocalArgs::shape of the propagation launch */
   int backoff = 8;
   int thresholdPct = 100;
+  int heavy = -1;       /* hand-back frontier size; -1 = never           */
   bool useGraphs = true;
   std::vector<GraphProp *> dProps; /* per batch: graphs that have it */
   std::vector<int> counts;
-  std::vector<hipGraphExec_t> exec;
+  std::vector<hipGraphExec_t> exec;     /* per batch: pipelines + launch   */
+  std::vector<hipGraphExec_t> execPost; /* per batch: batch ends          */
+  int *dExit = nullptr;                 /* per graph: hand-back iteration */
+  int *hExit = nullptr;                 /* pinned copy                    */
   hipGraphExec_t resetExec = nullptr;
   std::vector<hipEvent_t> evStart, evEnd;
   std::vector<char> ran;
@@ -41,11 +54,12 @@ struct dlp_multi {
   GraphProp *hStep = nullptr;       /* pinned staging, G entries         */
 
   void drop_graphs() {
-    for (hipGraphExec_t &e : exec)
-      if (e) {
-        (void)hipGraphExecDestroy(e);
-        e = nullptr;
-      }
+    for (std::vector<hipGraphExec_t> *v : {&exec, &execPost})
+      for (hipGraphExec_t &e : *v)
+        if (e) {
+          (void)hipGraphExecDestroy(e);
+          e = nullptr;
+        }
     if (resetExec) (void)hipGraphExecDestroy(resetExec);
     resetExec = nullptr;
   }
@@ -59,6 +73,10 @@ struct dlp_multi {
     dStep = nullptr;
     if (hStep) (void)hipHostFree(hStep);
     hStep = nullptr;
+    if (dExit) (void)hipFree(dExit);
+    dExit = nullptr;
+    if (hExit) (void)hipHostFree(hExit);
+    hExit = nullptr;
     for (dlp_context *c : ctx) dlp_context_destroy(c);
     ctx.clear();
     for (hipEvent_t e : evStart) (void)hipEventDestroy(e);
@@ -73,11 +91,11 @@ struct dlp_multi {
 namespace {
 
 /* per-graph descriptors of batch b's propagation launch */
-GraphProp graph_prop(const dlp_multi *m, const dlp_context *c);
+GraphProp graph_prop(const dlp_multi *m, int g);
 
 int upload_props(dlp_multi *m) {
   m->props.clear();
-  for (dlp_context *c : m->ctx) m->props.push_back(graph_prop(m, c));
+  for (int g = 0; g < m->G; ++g) m->props.push_back(graph_prop(m, g));
   for (int b = 0; b < m->maxB; ++b) {
     std::vector<GraphProp> h;
     for (int gi = 0; gi < m->G; ++gi) {
@@ -98,7 +116,8 @@ int upload_props(dlp_multi *m) {
 }
 
 /* the propagation descriptor of one graph (the same for every batch) */
-GraphProp graph_prop(const dlp_multi *m, const dlp_context *c) {
+GraphProp graph_prop(const dlp_multi *m, int gi) {
+  const dlp_context *c = m->ctx[gi];
   GraphProp g{};
   g.a = dlp_ctx_prop_args(c);
   g.l = c->local;
@@ -109,30 +128,61 @@ GraphProp graph_prop(const dlp_multi *m, const dlp_context *c) {
       m->thresholdPct < 0 ? 0 : (int)std::max<long long>(1, std::min<long long>(th, 1 << 30));
   if (c->N >= (1 << 30)) g.threshold = 0;  // outside-vertex encoding limit
   g.backoff = m->backoff;
+  g.heavy = m->heavy > 0 ? m->heavy : 0;
+  g.exitIter = m->dExit + gi;
   return g;
 }
 
-/* enqueue batch b of every graph on the shared stream */
-int enqueue_batch(dlp_multi *m, int b) {
+/* batch b of every graph that has one, up to and including the propagation
+ * launch, on the shared stream */
+int enqueue_batch_pre(dlp_multi *m, int b) {
   int rc;
-  std::vector<DBatch> bts(m->G);
   for (int g = 0; g < m->G; ++g) {
     dlp_context *c = m->ctx[g];
     if (b >= c->B) continue;
-    bts[g] = dlp_ctx_make_batch(c, b);
+    const DBatch bt = dlp_ctx_make_batch(c, b);
     int tag = 0;
-    if ((rc = dlp_ctx_batch_pre(c, b, bts[g], &tag)) != DLP_OK) return rc;
+    if ((rc = dlp_ctx_batch_pre(c, b, bt, &tag)) != DLP_OK) return rc;
   }
-  if ((rc = dlp_launch_propagate_multi(m->dProps[b], m->counts[b], m->shape, m->ldsWords,
-                                       m->stream)) != DLP_OK)
-    return rc;
+  return dlp_launch_propagate_multi(m->dProps[b], m->counts[b], m->shape, m->ldsWords,
+                                    m->stream);
+}
+
+/* the batch ends of batch b */
+int enqueue_batch_post(dlp_multi *m, int b) {
+  int rc;
   for (int g = 0; g < m->G; ++g) {
     dlp_context *c = m->ctx[g];
     if (b >= c->B) continue;
-    if ((rc = dlp_ctx_batch_post(c, b, bts[g])) != DLP_OK) return rc;
+    if ((rc = dlp_ctx_batch_post(c, b, dlp_ctx_make_batch(c, b))) != DLP_OK) return rc;
   }
   return DLP_OK;
 }
+
+/* after a propagation launch over `graphs` (count n): the graphs it handed
+ * back, read once the launch has finished (none when hand-back is off, in
+ * which case nothing waits) */
+int handed_back(dlp_multi *m, const int *graphs, int n, std::vector<int> *handed) {
+  handed->clear();
+  if (m->heavy <= 0 || n == 0) return DLP_OK;
+  DLP_HIP(hipMemcpyAsync(m->hExit, m->dExit, sizeof(int) * m->G, hipMemcpyDeviceToHost,
+                         m->stream));
+  DLP_HIP(hipStreamSynchronize(m->stream));
+  for (int i = 0; i < n; ++i) {
+    const int g = graphs[i];
+    const int k = m->hExit[g];
+    if (k < 0) continue;
+    if (k > m->ctx[g]->cfg.maxIterations) {
+      dlp_set_errorf_("graph %d handed back at iteration %d", g, k);
+      return DLP_ERR_INTERNAL;
+    }
+    handed->push_back(g);
+  }
+  return DLP_OK;
+}
+
+/* the rest of a handed-back graph's propagation, on the shared stream */
+int continue_graph(dlp_multi *m, int g) { return dlp_ctx_run_propagation(m->ctx[g], m->hExit[g]); }
 
 int enqueue_reset(dlp_multi *m) {
   for (dlp_context *c : m->ctx) {
@@ -222,6 +272,7 @@ extern "C" int dlp_multi_create(dlp_multi **out, int numGraphs, const dlp_config
   m->dProps.assign(m->maxB, nullptr);
   m->counts.assign(m->maxB, 0);
   m->exec.assign(m->maxB, nullptr);
+  m->execPost.assign(m->maxB, nullptr);
   m->ran.assign(m->maxB, 0);
   m->evStart.resize(m->maxB);
   m->evEnd.resize(m->maxB);
@@ -233,6 +284,19 @@ extern "C" int dlp_multi_create(dlp_multi **out, int numGraphs, const dlp_config
       dlp_set_error_("hipEventCreate failed");
       return DLP_ERR_HIP;
     }
+  }
+  {
+    void *p = nullptr;
+    if (hipMalloc(&p, sizeof(int) * m->G) != hipSuccess ||
+        hipHostMalloc((void **)&m->hExit, sizeof(int) * m->G, hipHostMallocDefault) !=
+            hipSuccess) {
+      if (p) (void)hipFree(p);
+      m->release();
+      delete m;
+      dlp_set_error_("dlp_multi_create: allocation of the hand-back slots failed");
+      return DLP_ERR_NOMEM;
+    }
+    m->dExit = (int *)p;
   }
   MTRY(upload_props(m));
   {
@@ -270,9 +334,10 @@ extern "C" int dlp_multi_set_tuning(dlp_multi *m, const dlp_multi_tuning *t) {
     dlp_set_errorf_("localShape must be in [1, %d]", dlp_local_shape_count());
     return DLP_ERR_INVALID;
   }
-  if (t->thresholdPercent < -1 || t->backoff < 0 || t->useGraphs < 0 || t->useGraphs > 2) {
-    dlp_set_error_("dlp_multi_set_tuning: thresholdPercent < -1, negative backoff or "
-                   "useGraphs not in {0,1,2}");
+  if (t->thresholdPercent < -1 || t->backoff < 0 || t->useGraphs < 0 || t->useGraphs > 2 ||
+      t->heavyFrontier < -1) {
+    dlp_set_error_("dlp_multi_set_tuning: thresholdPercent < -1, negative backoff, "
+                   "useGraphs not in {0,1,2} or heavyFrontier < -1");
     return DLP_ERR_INVALID;
   }
   DLP_HIP(hipStreamSynchronize(m->stream));
@@ -280,8 +345,10 @@ extern "C" int dlp_multi_set_tuning(dlp_multi *m, const dlp_multi_tuning *t) {
   if (t->thresholdPercent) m->thresholdPct = t->thresholdPercent;
   if (t->backoff) m->backoff = t->backoff;
   if (t->useGraphs) m->useGraphs = t->useGraphs == 1;
+  if (t->heavyFrontier) m->heavy = t->heavyFrontier;
   m->drop_graphs();  // captured launches embed the shape
   m->exec.assign(m->maxB, nullptr);
+  m->execPost.assign(m->maxB, nullptr);
   const int rc = upload_props(m);
   if (rc != DLP_OK) return rc;
   host_reset(m);
@@ -298,6 +365,7 @@ extern "C" int dlp_multi_get_tuning(const dlp_multi *m, dlp_multi_tuning *t) {
   t->thresholdPercent = m->thresholdPct;
   t->backoff = m->backoff;
   t->useGraphs = m->useGraphs ? 1 : 2;
+  t->heavyFrontier = m->heavy;
   return DLP_OK;
 }
 
@@ -335,17 +403,30 @@ extern "C" int dlp_multi_run_batch(dlp_multi *m, int b) {
     /* the pipelines read host bookkeeping (mark tags, last CC batch) that
      * evolves the same way on every run from a reset, so the capture of the
      * first run is valid for every later one */
-    if ((rc = capture(m, &m->exec[b], [&] { return enqueue_batch(m, b); })) != DLP_OK)
+    if ((rc = capture(m, &m->exec[b], [&] { return enqueue_batch_pre(m, b); })) != DLP_OK)
       return rc;
-    justCaptured = true;  // the capture ran the host bookkeeping already
+    if ((rc = capture(m, &m->execPost[b], [&] { return enqueue_batch_post(m, b); })) != DLP_OK)
+      return rc;
+    justCaptured = true;  // the captures ran the host bookkeeping already
   }
   DLP_HIP(hipEventRecord(m->evStart[b], m->stream));
   if (m->useGraphs) {
     DLP_HIP(hipGraphLaunch(m->exec[b], m->stream));
+  } else if ((rc = enqueue_batch_pre(m, b)) != DLP_OK) {
+    return rc;
+  }
+  std::vector<int> members, handed;
+  for (int g = 0; g < m->G; ++g)
+    if (b < m->ctx[g]->B) members.push_back(g);
+  if ((rc = handed_back(m, members.data(), (int)members.size(), &handed)) != DLP_OK) return rc;
+  for (int g : handed)
+    if ((rc = continue_graph(m, g)) != DLP_OK) return rc;
+  if (m->useGraphs) {
+    DLP_HIP(hipGraphLaunch(m->execPost[b], m->stream));
     if (!justCaptured)
       for (dlp_context *c : m->ctx)
         if (b < c->B) dlp_ctx_advance_host(c, b, dlp_ctx_make_batch(c, b));
-  } else if ((rc = enqueue_batch(m, b)) != DLP_OK) {
+  } else if ((rc = enqueue_batch_post(m, b)) != DLP_OK) {
     return rc;
   }
   DLP_HIP(hipEventRecord(m->evEnd[b], m->stream));
@@ -433,6 +514,55 @@ extern "C" int dlp_multi_device_bytes(const dlp_multi *m, int64_t *bytes) {
 /* serving: the next batch of any subset of graphs, and an arrival-driven   */
 /* scheduler over it                                                        */
 /* ------------------------------------------------------------------------ */
+namespace {
+
+/* the next batch of `graphs` (validated) together.  With `done` set, it is
+ * called with the stream idle for the graphs whose batch has completed:
+ * first all that the propagation launch finished, then each handed-back
+ * graph on its own. */
+int run_next_impl(dlp_multi *m, const int *graphs, int count,
+                  const std::function<void(const std::vector<int> &)> *done) {
+  DLP_HIP(hipStreamSynchronize(m->stream));  // the staging buffer is free
+  int rc;
+  for (int i = 0; i < count; ++i) {
+    dlp_context *c = m->ctx[graphs[i]];
+    const int b = c->nextBatch;
+    int tag = 0;
+    if ((rc = dlp_ctx_batch_pre(c, b, dlp_ctx_make_batch(c, b), &tag)) != DLP_OK) return rc;
+    m->hStep[i] = m->props[graphs[i]];
+  }
+  DLP_HIP(hipMemcpyAsync(m->dStep, m->hStep, sizeof(GraphProp) * count, hipMemcpyHostToDevice,
+                         m->stream));
+  if ((rc = dlp_launch_propagate_multi(m->dStep, count, m->shape, m->ldsWords, m->stream)) !=
+      DLP_OK)
+    return rc;
+  std::vector<int> handed, finished;
+  if ((rc = handed_back(m, graphs, count, &handed)) != DLP_OK) return rc;
+  auto post = [&](int g) {
+    dlp_context *c = m->ctx[g];
+    return dlp_ctx_batch_post(c, c->nextBatch, dlp_ctx_make_batch(c, c->nextBatch));
+  };
+  for (int i = 0; i < count; ++i) {
+    if (std::find(handed.begin(), handed.end(), graphs[i]) != handed.end()) continue;
+    if ((rc = post(graphs[i])) != DLP_OK) return rc;
+    finished.push_back(graphs[i]);
+  }
+  if (done && !finished.empty()) {
+    DLP_HIP(hipStreamSynchronize(m->stream));
+    (*done)(finished);
+  }
+  for (int g : handed) {
+    if ((rc = continue_graph(m, g)) != DLP_OK || (rc = post(g)) != DLP_OK) return rc;
+    if (done) {
+      DLP_HIP(hipStreamSynchronize(m->stream));
+      (*done)(std::vector<int>{g});
+    }
+  }
+  return DLP_OK;
+}
+
+}  // namespace
+
 extern "C" int dlp_multi_run_next(dlp_multi *m, const int *graphs, int count) {
   if (!m || count < 0 || (count > 0 && !graphs)) {
     dlp_set_error_("dlp_multi_run_next: bad argument");
@@ -452,27 +582,7 @@ extern "C" int dlp_multi_run_next(dlp_multi *m, const int *graphs, int count) {
       return DLP_ERR_STATE;
     }
   }
-  DLP_HIP(hipStreamSynchronize(m->stream));  // the staging buffer is free
-  int rc;
-  std::vector<DBatch> bts(count);
-  for (int i = 0; i < count; ++i) {
-    dlp_context *c = m->ctx[graphs[i]];
-    const int b = c->nextBatch;
-    bts[i] = dlp_ctx_make_batch(c, b);
-    int tag = 0;
-    if ((rc = dlp_ctx_batch_pre(c, b, bts[i], &tag)) != DLP_OK) return rc;
-    m->hStep[i] = m->props[graphs[i]];
-  }
-  DLP_HIP(hipMemcpyAsync(m->dStep, m->hStep, sizeof(GraphProp) * count, hipMemcpyHostToDevice,
-                         m->stream));
-  if ((rc = dlp_launch_propagate_multi(m->dStep, count, m->shape, m->ldsWords, m->stream)) !=
-      DLP_OK)
-    return rc;
-  for (int i = 0; i < count; ++i) {
-    dlp_context *c = m->ctx[graphs[i]];
-    if ((rc = dlp_ctx_batch_post(c, c->nextBatch, bts[i])) != DLP_OK) return rc;
-  }
-  return DLP_OK;
+  return run_next_impl(m, graphs, count, nullptr);
 }
 
 namespace {
@@ -548,13 +658,20 @@ extern "C" int dlp_multi_serve(dlp_multi *m, const dlp_serve_request *reqs, int 
       dlp_context *c = m->ctx[graphs[0]];
       if ((rc = dlp_context_run_batch(c, c->nextBatch)) != DLP_OK) return rc;
       DLP_HIP(hipStreamSynchronize(c->stream));
+      completion[admitted[0]] = now();
     } else {
-      if ((rc = dlp_multi_run_next(m, graphs.data(), (int)graphs.size())) != DLP_OK) return rc;
-      DLP_HIP(hipStreamSynchronize(m->stream));
+      /* a request completes when its graph's batch does (a handed-back
+       * graph after the rest of the step) */
+      const std::function<void(const std::vector<int> &)> stamp =
+          [&](const std::vector<int> &done) {
+            const double t = now();
+            for (size_t i = 0; i < graphs.size(); ++i)
+              if (std::find(done.begin(), done.end(), graphs[i]) != done.end())
+                completion[admitted[i]] = t;
+          };
+      if ((rc = run_next_impl(m, graphs.data(), (int)graphs.size(), &stamp)) != DLP_OK) return rc;
     }
     ++nSteps;
-    const double done = now();
-    for (int idx : admitted) completion[idx] = done;
     std::vector<int> rest;
     rest.reserve(pending.size());
     size_t a = 0;

@@ -75,9 +75,9 @@ int collect_timed(PropTiming *t) {
 
 int dlp_run_propagation(const PropArgs &a, const LocalArgs *l, int maxIter,
                         const PropLoopConfig &cfg, DevCounters *pin, hipStream_t s,
-                        PropTiming *timing) {
+                        PropTiming *timing, int startK) {
   int rc;
-  int k = 0;
+  int k = startK;
   int chunk = 2;
   int localAfter = 0; /* no local launch before this iteration (after an overflow) */
   const int maxChunk = std::max(1, cfg.maxChunk);

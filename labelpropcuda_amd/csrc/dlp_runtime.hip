@@ -200,7 +200,7 @@ DBatch dlp_ctx_make_batch(const dlp_context *c, int b) {
 
 namespace {
 
-int run_propagation(dlp_context *c, const PropArgs &a, int maxIter) {
+int run_propagation(dlp_context *c, const PropArgs &a, int maxIter, int startK = 0) {
   PropLoopConfig cfg;
   cfg.groupLanes = c->tuning.propGroupLanes;
   cfg.blocks = c->tuning.propBlocks;
@@ -209,7 +209,7 @@ int run_propagation(dlp_context *c, const PropArgs &a, int maxIter) {
   /* the local kernel encodes outside vertices as -2 - (2 * id + 1) */
   const bool local = c->tuning.localThreshold > 0 && c->N < (1 << 30);
   return dlp_run_propagation(a, local ? &c->local : nullptr, maxIter, cfg, c->pinCtr, c->stream,
-                             c->profiling ? &c->timing : nullptr);
+                             c->profiling ? &c->timing : nullptr, startK);
 }
 
 int trace_pre_propagation(dlp_context *c, const DBatch &bt, int markTag) {
@@ -634,6 +634,10 @@ int dlp_ctx_batch_pre(dlp_context *c, int b, const DBatch &bt, int *markTag) {
     if ((rc = dlp_labels_no_edges(d, bt, s)) != DLP_OK) return rc;
   }
   return dlp_frontier0(d, c->N, s);
+}
+
+int dlp_ctx_run_propagation(dlp_context *c, int startK) {
+  return run_propagation(c, dlp_ctx_prop_args(c), c->cfg.maxIterations, startK);
 }
 
 PropArgs dlp_ctx_prop_args(const dlp_context *c) {

@@ -425,6 +425,14 @@ extern "C" int dlp_irlp_ex(const int *rowPtr, const int *colInd, const float *va
   WS_BUF(float, buffer, 24, numNodes);
   WS_BUF(int, flags, 25, kIrlpMaxChunk + 1);
   hipStream_t s = (hipStream_t)stream;
+  int numEntries = 0;
+  DLP_HIP(hipMemcpyAsync(&numEntries, rowPtr + numNodes, sizeof(int), hipMemcpyDeviceToHost, s));
+  DLP_HIP(hipStreamSynchronize(s));
+  if (numEntries < 0 || (numEntries > 0 && !colInd)) {
+    dlp_set_error_("dlp_irlp: rowPtr[numNodes] negative or colInd NULL");
+    return DLP_ERR_INVALID;
+  }
+  WS_BUF(int, colMasked, 26, numEntries);
   /* the second buffer starts as a copy of the labels (validation.cu:157-159) */
   if ((rc = dlp_k_copy_float(buffer, labels, numNodes, s)) != DLP_OK) return rc;
   IrlpArgs a;
@@ -436,6 +444,8 @@ extern "C" int dlp_irlp_ex(const int *rowPtr, const int *colInd, const float *va
   a.lab0 = labels;
   a.lab1 = buffer;
   a.flags = flags;
+  a.colMasked = colMasked;
+  a.numEntries = numEntries;
   a.numNodes = numNodes;
   a.delta = delta;
   return dlp_run_irlp(a, maxIterations, cfg, w->pinFlags, s, iterationsUsed, kernelMs);

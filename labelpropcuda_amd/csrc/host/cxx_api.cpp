@@ -124,6 +124,7 @@ struct dlp_inputs {
 };
 
 extern "C" void dlp_set_error_(const char *msg);  // defined in the runtime
+void dlp_set_errorf_(const char *fmt, ...);         // defined in the runtime
 
 extern "C" void dlp_config_default(dlp_config *cfg) {
   if (cfg) *cfg = toCConfig(DynLPConfig());
@@ -159,6 +160,72 @@ extern "C" int dlp_inputs_generate(dlp_inputs **out, int totalNodes, int avgDegr
   if (!ok) {
     delete in;
     dlp_set_error_("dlp_inputs_generate: invalid generator arguments");
+    return DLP_ERR_INVALID;
+  }
+  in->flat = flattenBatches(batches);
+  *out = in;
+  return DLP_OK;
+}
+
+extern "C" int dlp_write_graph_files(const dlp_graph_view *graph, const float *trueLabels,
+                                     const char *prefix) {
+  if (!graph || !trueLabels || !prefix || graph->numNodes < 1 || graph->numEdges < 0 ||
+      !graph->rowPtr || (graph->numEdges > 0 && (!graph->colInd || !graph->values))) {
+    dlp_set_error_("dlp_write_graph_files: bad argument");
+    return DLP_ERR_INVALID;
+  }
+  try {
+    CSRGraph g;
+    g.numNodes = graph->numNodes;
+    g.numEdges = graph->numEdges;
+    g.rowPtr.assign(graph->rowPtr, graph->rowPtr + graph->numNodes + 1);
+    g.colInd.assign(graph->colInd, graph->colInd + graph->numEdges);
+    g.values.assign(graph->values, graph->values + graph->numEdges);
+    std::vector<float> labels(trueLabels, trueLabels + graph->numNodes);
+    if (!writeCSRGraph(prefix, g, labels)) {
+      dlp_set_error_("dlp_write_graph_files: could not write the files");
+      return DLP_ERR_INVALID;
+    }
+  } catch (const std::bad_alloc &) {
+    dlp_set_error_("dlp_write_graph_files: out of host memory");
+    return DLP_ERR_NOMEM;
+  }
+  return DLP_OK;
+}
+
+extern "C" int dlp_inputs_from_graph_files(dlp_inputs **out, const char *prefix,
+                                           int numBatches, unsigned int seed,
+                                           float groundTruthFraction, float deletedFraction) {
+  if (!out || !prefix) {
+    dlp_set_error_("dlp_inputs_from_graph_files: NULL argument");
+    return DLP_ERR_INVALID;
+  }
+  *out = nullptr;
+  dlp_inputs *in = new (std::nothrow) dlp_inputs();
+  if (!in) {
+    dlp_set_error_("dlp_inputs_from_graph_files: out of host memory");
+    return DLP_ERR_NOMEM;
+  }
+  if (!readCSRGraph(prefix, in->graph, in->trueLabels)) {
+    delete in;
+    dlp_set_errorf_("dlp_inputs_from_graph_files: cannot read %s*.txt", prefix);
+    return DLP_ERR_INVALID;
+  }
+  DynLPConfig cfg;
+  cfg.totalNodes = in->graph.numNodes;
+  cfg.numBatches = numBatches;
+  cfg.seed = seed;
+  cfg.groundTruthFraction = groundTruthFraction;
+  cfg.deletedFraction = deletedFraction;
+  cfg.verbose = false;
+  bool saved = dlp_host::g_quiet;
+  dlp_host::g_quiet = true;
+  std::vector<BatchData> batches;
+  const bool ok = generateBatches(in->graph, in->trueLabels, cfg, batches);
+  dlp_host::g_quiet = saved;
+  if (!ok) {
+    delete in;
+    dlp_set_error_("dlp_inputs_from_graph_files: invalid batch generator arguments");
     return DLP_ERR_INVALID;
   }
   in->flat = flattenBatches(batches);

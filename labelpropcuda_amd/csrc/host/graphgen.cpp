@@ -216,41 +216,48 @@ bool generateSparseGraph(int totalNodes, int avgDegree, unsigned int seed,
   edgeListToCSR(edgeList, weightList, totalNodes, graph);
 
   if (!outputPrefix.empty()) {  // text dump (:217-274)
-    if (!ensureDirectory(outputPrefix.substr(0, outputPrefix.find_last_of("/")))) return false;
-    {
-      std::ofstream f(outputPrefix + "RowPtr.txt");
-      if (!f.is_open()) {
-        std::cout << "Error: Could not write RowPtr file.\n";
-        return false;
-      }
-      for (int i = 0; i <= graph.numNodes; ++i) f << graph.rowPtr[i] << "\n";
-    }
-    {
-      std::ofstream f(outputPrefix + "ColInd.txt");
-      if (!f.is_open()) {
-        std::cout << "Error: Could not write ColInd file.\n";
-        return false;
-      }
-      for (int i = 0; i < graph.numEdges; ++i) f << graph.colInd[i] << "\n";
-    }
-    {
-      std::ofstream f(outputPrefix + "Values.txt");
-      if (!f.is_open()) {
-        std::cout << "Error: Could not write Values file.\n";
-        return false;
-      }
-      f << std::fixed;
-      for (int i = 0; i < graph.numEdges; ++i) f << std::setprecision(6) << graph.values[i] << "\n";
-    }
-    {
-      std::ofstream f(outputPrefix + "TrueLabels.txt");
-      if (!f.is_open()) {
-        std::cout << "Error: Could not write TrueLabels file.\n";
-        return false;
-      }
-      for (int i = 0; i < totalNodes; ++i) f << i << " " << trueLabels[i] << "\n";
-    }
+    if (!writeCSRGraph(outputPrefix, graph, trueLabels)) return false;
     if (!dlp_host::g_quiet) std::cout << "CSR graph written to: " << outputPrefix << "*.txt\n";
+  }
+  return true;
+}
+
+bool writeCSRGraph(const std::string &prefix, const CSRGraph &graph,
+                   const std::vector<float> &trueLabels) {
+  const size_t slash = prefix.find_last_of("/");
+  if (slash != std::string::npos && !ensureDirectory(prefix.substr(0, slash))) return false;
+  {
+    std::ofstream f(prefix + "RowPtr.txt");
+    if (!f.is_open()) {
+      std::cout << "Error: Could not write RowPtr file.\n";
+      return false;
+    }
+    for (int i = 0; i <= graph.numNodes; ++i) f << graph.rowPtr[i] << "\n";
+  }
+  {
+    std::ofstream f(prefix + "ColInd.txt");
+    if (!f.is_open()) {
+      std::cout << "Error: Could not write ColInd file.\n";
+      return false;
+    }
+    for (int i = 0; i < graph.numEdges; ++i) f << graph.colInd[i] << "\n";
+  }
+  {
+    std::ofstream f(prefix + "Values.txt");
+    if (!f.is_open()) {
+      std::cout << "Error: Could not write Values file.\n";
+      return false;
+    }
+    f << std::fixed;
+    for (int i = 0; i < graph.numEdges; ++i) f << std::setprecision(6) << graph.values[i] << "\n";
+  }
+  {
+    std::ofstream f(prefix + "TrueLabels.txt");
+    if (!f.is_open()) {
+      std::cout << "Error: Could not write TrueLabels file.\n";
+      return false;
+    }
+    for (int i = 0; i < graph.numNodes; ++i) f << i << " " << trueLabels[i] << "\n";
   }
   return true;
 }

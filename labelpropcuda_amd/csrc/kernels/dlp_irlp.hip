@@ -8,13 +8,19 @@
  * The reference runs one 256-thread block per vertex and one enforcement
  * kernel per iteration, and copies a changed flag to the host after every
  * iteration.  Here:
- *   - one sweep kernel per iteration: G lanes (4 .. 64, a power of two)
- *     per vertex, grid-stride over consecutive vertices, so the row
- *     pointers, labels, status and ground-truth class of a wave are read in
- *     runs and a row's entries are contiguous lanes of one load; ground
- *     truth enforcement is fused into the same store;
+ *   - before the first iteration, one pass replaces every column that
+ *     points at a DELETED vertex by -1 in a copy of colInd (the neighbour
+ *     status never changes during IrLP), so a sweep reads no neighbour
+ *     status: the entry adds nothing either way;
+ *   - one sweep kernel per iteration over consecutive vertices, with ground
+ *     truth enforcement fused into the label store.  Default: one lane per
+ *     vertex (64 consecutive vertices per wave, so the per-vertex arrays are
+ *     read in coalesced runs and every lane's row loads and label gathers
+ *     are independent), rows of more than 16 entries taken by the whole wave
+ *     one at a time; or G lanes (4 .. 64) per vertex;
  *   - the row sums are the reference block's 256-slot tree
- *     (dlp::group_row_sums), so every label is the same float;
+ *     (dlp::lane_row_sums / dlp::group_row_sums), so every label is the same
+ *     float;
  *   - the changed flag of iteration i is word i+1 of a small per-chunk
  *     array; the kernel of iteration i returns at once when word i is zero
  *     (iteration i-1 changed nothing), so a chunk of iterations is queued
@@ -24,8 +30,7 @@
  *
  * Per iteration over N vertices and E stored entries the algorithmic
  * traffic is 20 B per vertex (row pointer, ground-truth class, status, old
- * and new label) and 16 B per entry (column, weight, the neighbour's status
- * and label).
+ * and new label) and 12 B per entry (column, weight, the neighbour's label).
  */
 #include <algorithm>
 
@@ -40,6 +45,85 @@ constexpr int kIrlpThreads = 256;
  * 1..C <- 0 */
 __global__ void k_irlp_flags(int *flags, int C) {
   for (int i = threadIdx.x; i <= C; i += blockDim.x) flags[i] = i == 0 ? 1 : 0;
+}
+
+/* colMasked[e] = colInd[e], or -1 where that is negative or a DELETED
+ * vertex */
+__global__ void k_irlp_mask(const int *colInd, const int *status, int *colMasked, long long E) {
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < E;
+       e += (long long)gridDim.x * blockDim.x) {
+    const int v = colInd[e];
+    colMasked[e] = (v >= 0 && status[v] != DLP_VERTEX_DELETED) ? v : -1;
+  }
+}
+
+/* the label update of vertex u from its row sums, with ground truth
+ * enforced (validation.cu:56-70, :107-126, :131-137) */
+__device__ __forceinline__ float irlp_label(int gt, bool live, float W, float S, float old,
+                                            float delta, bool &changed) {
+  if (gt == 0) return 0.0f;
+  if (gt == 1) return 1.0f;
+  if (!live) return old;
+  const float nl = dlp::new_label(W, S, old);
+  changed = changed || fabsf(nl - old) > delta;
+  return nl;
+}
+
+constexpr int kIrlpLaneRowMax = 16;
+
+/* one lane per vertex, 64 consecutive vertices per wave; iteration `iter`
+ * as word j of the chunk */
+__global__ __launch_bounds__(kIrlpThreads) void k_irlp_lanes(IrlpArgs a, int iter, int j) {
+  if (a.flags[j] == 0) return;  // converged before this iteration (grid-uniform)
+  const float *Fold = (iter & 1) ? a.lab1 : a.lab0;
+  float *Fnew = (iter & 1) ? a.lab0 : a.lab1;
+  const int *col = a.colMasked ? a.colMasked : a.colInd;
+  const int *nbStatus = a.colMasked ? nullptr : a.status;
+  const int lane = threadIdx.x & 63;
+  const long long gwave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const long long nwaves = ((long long)gridDim.x * blockDim.x) >> 6;
+  const long long N = a.numNodes;
+  bool changed = false;
+  for (long long base = gwave * 64; base < N; base += nwaves * 64) {  // wave-uniform
+    const long long u = base + lane;
+    const bool act = u < N;
+    int gt = -1, st = DLP_VERTEX_DELETED, beg = 0, len = 0;
+    if (act) {
+      gt = a.gtClass[u];
+      st = a.status[u];
+    }
+    const bool live = act && gt < 0 && st == DLP_VERTEX_ACTIVE;
+    if (live) {
+      beg = a.rowPtr[u];
+      len = a.rowPtr[u + 1] - beg;
+    }
+    const bool laneRow = live && len <= kIrlpLaneRowMax;
+    const int lenS = laneRow ? len : 0;
+    const dlp::GlobalPos pos{col, a.values, Fold, nbStatus, beg};
+    float W, S;
+    if (!__any(lenS > 4))
+      dlp::lane_row_sums<4>(lenS, pos, W, S);
+    else if (!__any(lenS > 8))
+      dlp::lane_row_sums<8>(lenS, pos, W, S);
+    else
+      dlp::lane_row_sums<kIrlpLaneRowMax>(lenS, pos, W, S);
+    if (act && (laneRow || !live)) Fnew[u] = irlp_label(gt, live, W, S, Fold[u], a.delta, changed);
+    /* longer rows: the wave takes them one by one */
+    unsigned long long longMask = dlp_ballot(live && !laneRow);
+    while (longMask) {  // wave-uniform
+      const int src = __ffsll((long long)longMask) - 1;
+      longMask &= longMask - 1;
+      const int lbeg = __shfl(beg, src, 64), llen = __shfl(len, src, 64);
+      float LW, LS;
+      dlp::group_row_sums<64>(lane, llen, dlp::GlobalPos{col, a.values, Fold, nbStatus, lbeg},
+                              LW, LS);
+      if (lane == 0) {
+        const long long lu = base + src;
+        Fnew[lu] = irlp_label(-1, true, LW, LS, Fold[lu], a.delta, changed);
+      }
+    }
+  }
+  if (__any(changed) && lane == 0) a.flags[j + 1] = 1;
 }
 
 /* iteration `iter` (global index, picks the buffers) as word j of the
@@ -73,31 +157,21 @@ __global__ __launch_bounds__(kIrlpThreads) void k_irlp(IrlpArgs a, int iter, int
       len = a.rowPtr[u + 1] - beg;
     }
     float W, S;
-    dlp::group_row_sums<G>(gl, len, dlp::GlobalPos{a.colInd, a.values, Fold, a.status, beg}, W,
-                           S);
-    if (act && gl == 0) {
-      float nl;
-      if (gt == 0) {
-        nl = 0.0f;
-      } else if (gt == 1) {
-        nl = 1.0f;
-      } else {
-        const float old = Fold[u];
-        nl = old;
-        if (live) {  // validation.cu:107-126
-          nl = dlp::new_label(W, S, old);
-          changed = changed || fabsf(nl - old) > a.delta;
-        }
-      }
-      Fnew[u] = nl;
-    }
+    dlp::group_row_sums<G>(gl, len,
+                           dlp::GlobalPos{a.colMasked ? a.colMasked : a.colInd, a.values, Fold,
+                                          a.colMasked ? nullptr : a.status, beg},
+                           W, S);
+    if (act && gl == 0) Fnew[u] = irlp_label(gt, live, W, S, Fold[u], a.delta, changed);
   }
   if (__any(changed) && lane == 0) a.flags[j + 1] = 1;
 }
 
 template <int G>
 int launch_irlp(const IrlpArgs &a, int iter, int j, int blocks, hipStream_t s) {
-  k_irlp<G><<<blocks, kIrlpThreads, 0, s>>>(a, iter, j);
+  if (G == 1)
+    k_irlp_lanes<<<blocks, kIrlpThreads, 0, s>>>(a, iter, j);
+  else
+    k_irlp<G><<<blocks, kIrlpThreads, 0, s>>>(a, iter, j);
   return DLP_OK;
 }
 
@@ -115,8 +189,8 @@ int dlp_run_irlp(const IrlpArgs &a, int maxIterations, const IrlpLoopConfig &cfg
   if (kernelMs) *kernelMs = 0.0f;
   if (maxIterations <= 0 || a.numNodes <= 0) return DLP_OK;
   const int G = cfg.groupLanes;
-  if (G != 4 && G != 8 && G != 16 && G != 32 && G != 64) {
-    dlp_set_errorf_("IrLP: group lanes %d not in {4,8,16,32,64}", G);
+  if (G != 1 && G != 4 && G != 8 && G != 16 && G != 32 && G != 64) {
+    dlp_set_errorf_("IrLP: group lanes %d not in {1,4,8,16,32,64}", G);
     return DLP_ERR_INVALID;
   }
   if (cfg.firstChunk < 1 || cfg.maxChunk < cfg.firstChunk || cfg.maxChunk > kIrlpMaxChunk) {
@@ -130,6 +204,10 @@ int dlp_run_irlp(const IrlpArgs &a, int maxIterations, const IrlpLoopConfig &cfg
     DLP_HIP(hipEventCreate(&ev[1]));
     DLP_HIP(hipEventRecord(ev[0], s));
   }
+  if (a.colMasked && a.numEntries > 0) {
+    const long long blocksM = std::min<long long>((a.numEntries + 255) / 256, 8192);
+    k_irlp_mask<<<(int)blocksM, 256, 0, s>>>(a.colInd, a.status, a.colMasked, a.numEntries);
+  }
   int used = -1, it = 0, chunk = cfg.firstChunk;
   int rc = DLP_OK;
   while (it < maxIterations && used < 0 && rc == DLP_OK) {
@@ -137,6 +215,7 @@ int dlp_run_irlp(const IrlpArgs &a, int maxIterations, const IrlpLoopConfig &cfg
     k_irlp_flags<<<1, 64, 0, s>>>(a.flags, C);
     for (int j = 0; j < C && rc == DLP_OK; ++j) {
       switch (G) {
+        case 1: rc = launch_irlp<1>(a, it + j, j, blocks, s); break;
         case 4: rc = launch_irlp<4>(a, it + j, j, blocks, s); break;
         case 8: rc = launch_irlp<8>(a, it + j, j, blocks, s); break;
         case 16: rc = launch_irlp<16>(a, it + j, j, blocks, s); break;

@@ -869,12 +869,13 @@ __device__ __forceinline__ void wg_lane_rows(const PropArgs &a, const float *Fol
 }
 
 /* Iterations k, k+1, ... by one work-group over HBM, while the frontier is
- * non-empty, k < maxIter, and not (frontier <= threshold and k >= resumeAt):
- * the same global state transitions as one k_propagate launch each.  On
- * return S.misc[0] holds the frontier size of iteration k. */
+ * non-empty, k < maxIter, not (frontier <= threshold and k >= resumeAt) and
+ * not (heavy > 0 and frontier > heavy): the same global state transitions as
+ * one k_propagate launch each.  On return S.misc[0] holds the frontier size
+ * of iteration k. */
 template <int LT>
 __device__ __forceinline__ void wg_iterations(const PropArgs &a, int &k, int maxIter, int threshold,
-                              int resumeAt, LocalShared &S, LocalProf &prof) {
+                              int resumeAt, int heavy, LocalShared &S, LocalProf &prof) {
   constexpr int kWgLaneRowMax = 16;
   DevCounters *c = a.ctr;
   const int tid = threadIdx.x, lane = tid & 63;
@@ -884,7 +885,8 @@ __device__ __forceinline__ void wg_iterations(const PropArgs &a, int &k, int max
     if (tid == 0) S.misc[0] = ld_agent(&c->qcount[k % 3]);
     __syncthreads();
     const int n = S.misc[0];
-    if (n == 0 || k >= maxIter || (n <= threshold && k >= resumeAt)) break;  // uniform
+    if (n == 0 || k >= maxIter || (n <= threshold && k >= resumeAt) || (heavy > 0 && n > heavy))
+      break;  // uniform
     if (tid == 0) {
       c->qcount[(k + 2) % 3] = 0;
       if (k == 0) c->frontier0 = n;
@@ -984,9 +986,12 @@ __device__ __forceinline__ void wg_prof_flush(const PropArgs &a, LocalProf &prof
   prof = LocalProf{};
 }
 
-/* One work-group per graph runs the whole propagation of its graph's batch:
+/* One work-group per graph runs the propagation of its graph's batch:
  * LDS-resident passes while the frontier is at most the threshold (and no
- * overflow in the last `backoff` iterations), HBM iterations otherwise. */
+ * overflow in the last `backoff` iterations), HBM iterations otherwise.  A
+ * frontier above `heavy` (when > 0) ends the graph's work at the start of
+ * that iteration, with the state a multi-block launch would leave, and
+ * *exitIter tells the host where to resume; otherwise *exitIter = -1. */
 template <int LT, int SR>
 __global__ __launch_bounds__(LT) void k_propagate_multi(const GraphProp *graphs) {
   extern __shared__ int4 pool4[];
@@ -997,22 +1002,30 @@ __global__ __launch_bounds__(LT) void k_propagate_multi(const GraphProp *graphs)
   const int maxIter = gp.maxIter, threshold = gp.threshold, backoff = gp.backoff;
   if (threadIdx.x < 64) S.dummyStamp[threadIdx.x] = DLP_INT_MAX;
   LocalProf lprof, gprof;
-  int k = 0, resumeAt = 0;
+  const int heavy = gp.heavy;
+  int k = 0, resumeAt = 0, handBack = -1;
   if (l.ldsWords > 0 && threshold > 0) {
     while (true) {
-      wg_iterations<LT>(a, k, maxIter, threshold, resumeAt, S, gprof);
+      wg_iterations<LT>(a, k, maxIter, threshold, resumeAt, heavy, S, gprof);
       const int n = S.misc[0];
       if (n == 0 || k >= maxIter) break;  // uniform
+      if (heavy > 0 && n > heavy) {
+        handBack = k;
+        break;
+      }
       const int reason = local_passes<LT, SR>(a, l, k, maxIter, (int *)pool4, S, lprof);
       if (reason != DLP_LOCAL_TOO_BIG) break;
       resumeAt = k + backoff;
       wg_global_sync();
     }
   } else {
-    wg_iterations<LT>(a, k, maxIter, -1, maxIter, S, gprof);
+    wg_iterations<LT>(a, k, maxIter, -1, maxIter, heavy, S, gprof);
+    const int n = S.misc[0];
+    if (n > 0 && k < maxIter && heavy > 0 && n > heavy) handBack = k;
   }
   local_prof_flush(a, lprof);
   wg_prof_flush(a, gprof);
+  if (threadIdx.x == 0 && gp.exitIter) *gp.exitIter = handBack;
 }
 
 }  // namespace

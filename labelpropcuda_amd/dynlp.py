@@ -133,6 +133,25 @@ def generate_stream(totalNodes: int, avgDegree: int, batchSizes, seed: int = 42,
     return _inputs_from_handle(lib, h)
 
 
+def write_graph(inputs: Inputs, prefix: str) -> None:
+    """The generator's CSR text dump of `inputs`' full graph and true labels
+    (<prefix>RowPtr.txt, ColInd.txt, Values.txt, TrueLabels.txt)."""
+    g = inputs.graph_view()
+    check(load().dlp_write_graph_files(C.byref(g), _ptr(inputs.trueLabels, C.c_float),
+                                       prefix.encode()))
+
+
+def from_graph_files(prefix: str, numBatches: int, seed: int = 42, gtFraction: float = 0.01,
+                     deletedFraction: float = 0.09) -> Inputs:
+    """A graph read back from the CSR text dump, with the reference batch
+    generator run over it."""
+    lib = load()
+    h = C.c_void_p()
+    check(lib.dlp_inputs_from_graph_files(C.byref(h), prefix.encode(), numBatches, seed,
+                                          gtFraction, deletedFraction))
+    return _inputs_from_handle(lib, h)
+
+
 def _inputs_from_handle(lib, h) -> Inputs:
     try:
         g = _lib.GraphView()
@@ -445,9 +464,11 @@ class Multi:
         except Exception:
             pass
 
-    def set_tuning(self, localShape=0, thresholdPercent=0, backoff=0, useGraphs=0):
-        """0 keeps a setting; useGraphs 1 = HIP-graph replay, 2 = enqueue."""
-        t = _lib.MultiTuning(localShape, thresholdPercent, backoff, useGraphs)
+    def set_tuning(self, localShape=0, thresholdPercent=0, backoff=0, useGraphs=0,
+                   heavyFrontier=0):
+        """0 keeps a setting; useGraphs 1 = HIP-graph replay, 2 = enqueue;
+        heavyFrontier -1 = graphs never leave the shared launch."""
+        t = _lib.MultiTuning(localShape, thresholdPercent, backoff, useGraphs, heavyFrontier)
         check(self._lib.dlp_multi_set_tuning(self._h, C.byref(t)))
 
     def tuning(self) -> dict:

@@ -11,6 +11,7 @@
 #include "dynlp_oracle.h"
 
 #include <math.h>
+#include <omp.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -298,6 +299,73 @@ static void row_sums(const int *colInd, const float *values, int beg, int end,
  * frontierSizes (nullable) receives numAffected of every processed
  * iteration; isAffected ends as the reference leaves it (the last
  * iteration's marks, or unchanged when no iteration ran). */
+/* Worker threads of the frontier-parallel variant of propagate (1 = the
+ * literal sequential loop).  With more threads the label updates of a
+ * frontier run in parallel (each reads only the old buffer, so the floats
+ * are the same) and the next frontier is collected per thread and
+ * concatenated: the same vertex set in another order, so labels, frontier
+ * sizes, iteration counts and isAffected are unchanged. */
+static int g_threads = 1;
+static int g_parMin = 1024; /* smallest frontier run in parallel */
+void FN(set_threads)(int threads, int minFrontier) {
+  g_threads = threads < 1 ? 1 : threads;
+  g_parMin = minFrontier < 1 ? 1 : minFrontier;
+}
+
+/* thread-local mark buffers of the parallel variant */
+typedef struct {
+  int *v;
+  int n, cap;
+} MarkBuf;
+
+static void markbuf_push(MarkBuf *b, int v) {
+  if (b->n == b->cap) {
+    b->cap = b->cap ? 2 * b->cap : 1024;
+    b->v = (int *)realloc(b->v, sizeof(int) * (size_t)b->cap);
+  }
+  b->v[b->n++] = v;
+}
+
+/* marks of the changed frontier vertices, in parallel (markNeighborsKernel,
+ * iterativePropagation.cu:187-217): each thread collects the vertices it
+ * marked first, then the per-thread lists are concatenated into `marks`.
+ * bufs and offs hold g_threads and g_threads + 1 entries.  Returns the
+ * number of marks. */
+static int mark_parallel(const int *rowPtr, const int *colInd, const int *status,
+                         const int *gtClass, const int *list, int n, const real *nw,
+                         const real *old, real d, char *marked, int *marks, MarkBuf *bufs,
+                         int *offs) {
+  int total = 0;
+#pragma omp parallel num_threads(g_threads)
+  {
+    const int t = omp_get_thread_num();
+    MarkBuf *b = &bufs[t];
+    b->n = 0;
+#pragma omp for schedule(dynamic, 256)
+    for (int i = 0; i < n; ++i) {
+      const int u = list[i];
+      if (!(absr(nw[u] - old[u]) > d)) continue;
+      for (int e = rowPtr[u]; e < rowPtr[u + 1]; ++e) {
+        const int v = colInd[e];
+        if (v >= 0 && status[v] != V_DELETED && gtClass[v] < 0 &&
+            !__atomic_exchange_n(&marked[v], 1, __ATOMIC_RELAXED))
+          markbuf_push(b, v);
+      }
+      if (gtClass[u] < 0 && !__atomic_exchange_n(&marked[u], 1, __ATOMIC_RELAXED))
+        markbuf_push(b, u);
+    }
+#pragma omp single
+    {
+      const int T = omp_get_num_threads();
+      offs[0] = 0;
+      for (int k = 0; k < T; ++k) offs[k + 1] = offs[k] + bufs[k].n;
+      total = offs[T];
+    }
+    if (b->n) memcpy(marks + offs[t], b->v, sizeof(int) * (size_t)b->n);
+  }
+  return total;
+}
+
 int FN(propagate)(const int *rowPtr, const int *colInd, const float *values,
                   real *labels, real *labelsBuffer, const int *status,
                   const int *gtClass, int *isAffected, int numNodes,
@@ -307,6 +375,8 @@ int FN(propagate)(const int *rowPtr, const int *colInd, const float *values,
   int *list = (int *)malloc(sizeof(int) * N1);
   int *marks = (int *)malloc(sizeof(int) * N1);
   char *marked = (char *)calloc(N1, 1);
+  MarkBuf *bufs = (MarkBuf *)calloc((size_t)g_threads, sizeof(MarkBuf));
+  int *offs = (int *)malloc(sizeof(int) * (size_t)(g_threads + 1));
   int numMarks = 0;
   real *old = labels, *nw = labelsBuffer;
   const real d = (real)delta;
@@ -328,7 +398,10 @@ int FN(propagate)(const int *rowPtr, const int *colInd, const float *values,
     if (frontierSizes && nfs < maxFrontierSizes) frontierSizes[nfs] = n;
     ++nfs;
     int changed = 0;
+    const int par = g_threads > 1 && n >= g_parMin;
     /* propagateLabelsKernel (:98-179) */
+#pragma omp parallel for num_threads(g_threads) if (par) reduction(| : changed) \
+    schedule(dynamic, 256)
     for (int i = 0; i < n; ++i) {
       int u = list[i];
       real W, S;
@@ -342,6 +415,14 @@ int FN(propagate)(const int *rowPtr, const int *colInd, const float *values,
     /* markNeighborsKernel (:187-217) into cleared flags (:299-305) */
     for (int i = 0; i < numMarks; ++i) marked[marks[i]] = 0;
     numMarks = 0;
+    if (par) {
+      numMarks = mark_parallel(rowPtr, colInd, status, gtClass, list, n, nw, old, d, marked,
+                               marks, bufs, offs);
+      real *tmp = old; old = nw; nw = tmp;
+      iterationsUsed = iter + 1;
+      if (!changed) break;
+      continue;
+    }
     for (int i = 0; i < n; ++i) {
       int u = list[i];
       if (absr(nw[u] - old[u]) > d) {
@@ -375,6 +456,9 @@ int FN(propagate)(const int *rowPtr, const int *colInd, const float *values,
   free(list);
   free(marks);
   free(marked);
+  for (int t = 0; t < g_threads; ++t) free(bufs[t].v);
+  free(bufs);
+  free(offs);
   return iterationsUsed;
 }
 

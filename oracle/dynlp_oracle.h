@@ -55,6 +55,9 @@ extern "C" {
                      int numNodes, float delta, int maxIterations,              \
                      int *frontierSizes, int maxFrontierSizes,                  \
                      int *numFrontierSizes);                                    \
+  /* worker threads of propagate (1 = sequential; more: same results) for     \
+   * frontiers of at least minFrontier vertices */                             \
+  void PFX##set_threads(int threads, int minFrontier);                          \
   int PFX##irlp(const int *rowPtr, const int *colInd, const float *values,      \
                  REAL *labels, const int *status, const int *gtClass,           \
                  int numNodes, float delta, int maxIterations);                 \

@@ -51,6 +51,7 @@ def load() -> C.CDLL:
             "propagate": (C.c_int, [i32p, i32p, f32p, rp, rp, i32p, i32p, i32p, C.c_int,
                                     C.c_float, C.c_int, i32p, C.c_int, i32p]),
             "irlp": (C.c_int, [i32p, i32p, f32p, rp, i32p, i32p, C.c_int, C.c_float, C.c_int]),
+            "set_threads": (None, [C.c_int, C.c_int]),
             "create": (C.c_void_p, [C.c_int, i32p, i32p, f32p, f32p, C.c_int, i64p, i32p, i64p,
                                     i32p, f32p, i64p, i32p, i64p, i32p, i32p, f32p, C.c_float,
                                     C.c_float, C.c_int]),
@@ -74,6 +75,15 @@ def load() -> C.CDLL:
             fn.argtypes = args
     _lib = lib
     return lib
+
+
+def set_threads(threads: int, minFrontier: int = 1024) -> None:
+    """Worker threads of the propagation loop (1 = the literal sequential
+    restatement; more = the frontier-parallel variant, same results) for
+    frontiers of at least minFrontier vertices."""
+    lib = load()
+    lib.orc32_set_threads(int(threads), int(minFrontier))
+    lib.orc64_set_threads(int(threads), int(minFrontier))
 
 
 def _p(a: np.ndarray):

@@ -29,7 +29,8 @@ def final_states():
 
 
 @pytest.mark.parametrize("groupLanes,firstChunk,maxChunk",
-                         [(8, 8, 64), (4, 1, 1), (16, 3, 5), (64, 256, 256), (32, 2, 200)])
+                         [(1, 8, 64), (1, 1, 1), (8, 8, 64), (4, 1, 1), (16, 3, 5),
+                          (64, 256, 256), (32, 2, 200)])
 def test_irlp_matches_oracle(gpu_available, final_states, groupLanes, firstChunk, maxChunk):
     t = L.dynlp.irlp_tuning(groupLanes=groupLanes, firstChunk=firstChunk, maxChunk=maxChunk)
     for key, (_, (rp, ci, vv, st, gt, lab)) in final_states.items():
@@ -61,9 +62,11 @@ def test_irlp_random_states_with_holes(gpu_available):
         labels[gtClass >= 0] = 0.25  # enforcement must overwrite these
         for delta in (1e-4, 0.0):
             exp, ei = O.irlp(rowPtr, col, values, labels, status, gtClass, delta, 500)
-            got, gi, _ = L.dynlp.irlp(rowPtr, col, values, labels, status, gtClass, delta, 500)
-            assert gi == ei
-            assert np.array_equal(bits(got), bits(exp))
+            for g in (1, 8):
+                got, gi, _ = L.dynlp.irlp(rowPtr, col, values, labels, status, gtClass, delta,
+                                          500, tuning=L.dynlp.irlp_tuning(groupLanes=g))
+                assert gi == ei
+                assert np.array_equal(bits(got), bits(exp))
 
 
 def test_irlp_validation_host_arrays(gpu_available, final_states):

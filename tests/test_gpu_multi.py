@@ -79,6 +79,22 @@ def test_multi_engine_modes(gpu_available, small_graphs, threshold, graphs_mode)
         check_multi(m, small_graphs, plain)
 
 
+@pytest.mark.parametrize("heavy,graphs_mode", [(1, 1), (100, 1), (100, 2), (-1, 1)])
+def test_multi_hand_back(gpu_available, small_graphs, heavy, graphs_mode):
+    """Graphs whose frontier outgrows `heavy` leave the shared launch and
+    finish on the single-graph loop (1: almost every graph at its first
+    iteration; -1: never); two steps, so replays are covered too."""
+    plain = plain_results(small_graphs, 3000)
+    cfgs = [L.make_config(g, maxIterations=3000) for g in small_graphs]
+    with L.Multi(small_graphs, cfgs) as m:
+        m.set_tuning(heavyFrontier=heavy, useGraphs=graphs_mode)
+        assert m.tuning()["heavyFrontier"] == heavy
+        for _ in range(2):
+            m.run_all()
+            m.synchronize()
+            check_multi(m, small_graphs, plain)
+
+
 def test_multi_matches_oracle(gpu_available):
     graphs = [L.generate(3000, 5, 5, 11), L.generate(6000, 3, 5, 12),
               L.generate(2500, 7, 5, 13)]

@@ -44,17 +44,22 @@ def requests(tenants, rng, spread):
     return sorted(reqs, key=lambda r: r[1])
 
 
-@pytest.mark.parametrize("slots,budget,plain,spread", [
-    (64, 0, False, 0.0),        # everything at once: maximal mixing
-    (3, 0, False, 0.05),        # few slots, staggered arrivals
-    (64, 4096, False, 0.0),     # tiny HBM budget: one batch per step
-    (1, 0, True, 0.0),          # plain: one batch at a time, single-graph path
+@pytest.mark.parametrize("slots,budget,plain,spread,heavy", [
+    (64, 0, False, 0.0, 0),        # everything at once: maximal mixing
+    (3, 0, False, 0.05, 0),        # few slots, staggered arrivals
+    (64, 4096, False, 0.0, 0),     # tiny HBM budget: one batch per step
+    (1, 0, True, 0.0, 0),          # plain: one batch at a time, single-graph path
+    (64, 0, False, 0.0, 1),        # every step hands its graphs back
+    (4, 0, False, 0.02, 200),      # some graphs handed back, staggered
+    (64, 0, False, 0.0, -1),       # no hand-back
 ])
-def test_serving_same_alone_and_in_mix(gpu_available, tenants, slots, budget, plain, spread):
+def test_serving_same_alone_and_in_mix(gpu_available, tenants, slots, budget, plain, spread,
+                                       heavy):
     ref = alone(tenants, 3000)
     cfgs = [L.make_config(t, maxIterations=3000) for t in tenants]
     rng = np.random.default_rng(11)
     with L.Multi(tenants, cfgs) as m:
+        m.set_tuning(heavyFrontier=heavy)
         m.reset()
         m.run_next(list(range(len(tenants))))  # seed batches: tenants become resident
         reqs = requests(tenants, rng, spread)

@@ -310,3 +310,25 @@ def test_fp64_twin_error_is_small(c1_inputs):
     err = np.abs(o32.labels().astype(np.float64) - o64.labels())
     # threshold decisions may differ, so this is a statistic, bounded loosely
     assert err.mean() < 1e-3 and err.max() < 0.1
+
+
+@pytest.mark.parametrize("fp64", [False, True])
+def test_parallel_propagation_gives_identical_results(c1_inputs, fp64):
+    """The frontier-parallel variant (the multi-core CPU baseline) against
+    the sequential restatement: same labels, frontier sizes, iteration and
+    update counts, accuracy."""
+    def run(threads):
+        O.set_threads(threads, minFrontier=1)
+        try:
+            o = O.Oracle(c1_inputs, fp64=fp64)
+            st = []
+            for b in range(c1_inputs.numBatches):
+                o.run_batch(b)
+                st.append((o.stats(), o.frontier_sizes().tolist(), o.affected().tolist()))
+            return o.labels(), st
+        finally:
+            O.set_threads(1)
+    lab1, st1 = run(1)
+    lab4, st4 = run(4)
+    assert same_bits(lab1, lab4)
+    assert st1 == st4
