@@ -66,6 +66,43 @@ def engine_rooflines(prof: dict, updates: int) -> dict:
     return out
 
 
+def random_access_ceilings(array_ints: int, accesses: int = 1 << 24) -> dict:
+    """Random 4-byte accesses per second the GPU sustains over an array of
+    the workload's per-vertex size (dlp_random_access_rates)."""
+    import ctypes as C
+    from labelpropcuda_amd import _lib
+    rates = (C.c_double * 3)()
+    _lib.check(_lib.load().dlp_random_access_rates(array_ints, accesses, rates))
+    return {"array_ints": array_ints, "atomic_max_per_s": rates[0], "load_per_s": rates[1],
+            "store_per_s": rates[2]}
+
+
+def random_access_floor_us(prof: dict, updates: int, launches: float, ra: dict) -> float:
+    """Time per multi-block launch if its random accesses ran at the measured
+    rates back to back: per vertex update the row start, row length and old
+    label loads and the new-label store, per row entry the neighbour-label
+    load, per claim (a changed vertex's entries and itself) one returning
+    atomicMax."""
+    u = updates - prof["localUpdates"]
+    loads = 3.0 * u + prof["edges"]
+    t = loads / ra["load_per_s"] + u / ra["store_per_s"] + prof["changedEdges"] / ra["atomic_max_per_s"]
+    return 1e6 * t / max(launches, 1.0)
+
+
+def add_random_access(engines: dict, prof: dict, updates: int, array_ints: int,
+                      out: dict) -> dict:
+    """Adds the multi-block kernel's random-access floor (and its ratio to
+    the measured launch time) and records the measured rates in out."""
+    ra = random_access_ceilings(array_ints)
+    out["random_access"] = ra
+    k = engines.get("k_propagate")
+    if k:
+        floor = random_access_floor_us(prof, updates, k["launches_per_step"], ra)
+        k["random_access_floor_us"] = floor
+        k["random_access_frac"] = floor / k["avg_launch_us"] if k["avg_launch_us"] else None
+    return engines
+
+
 def measured_traffic(kernel: str):
     """HBM bytes per launch of `kernel` on this workload, from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE summary (None when absent)."""
@@ -281,14 +318,16 @@ def serving_main(args) -> int:
         m.synchronize()
         reqs = list(zip(reqTenant.tolist(), arrivals.tolist()))
         done, steps = m.serve(reqs, maxSlots=args.slots, hbmBudgetBytes=args.hbm_budget,
-                              plain=plain)
+                              plain=plain, handBackLanes=args.lanes_back)
         m.synchronize()
         lat = 1000.0 * (done - arrivals)
+        assert np.all(lat >= 0.0), "a request completed before it arrived"
         upd = sum(m.stats(t, b)["updates"] for t in range(T)
                   for b in range(1, tenants[t].numBatches))
         return {"makespan_s": float(done.max()), "steps": steps, "updates": int(upd),
                 "updates_per_s": upd / float(done.max()),
-                "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99))}
+                "min_ms": float(lat.min()), "p50_ms": float(np.percentile(lat, 50)),
+                "p99_ms": float(np.percentile(lat, 99))}
 
     zero = np.zeros(R)
     sat_plain = run(zero, True)
@@ -420,7 +459,8 @@ def stream_main(args) -> int:
                                 "local_launches": timed["localLaunches"]},
     }
     if prof is not None:
-        out["propagation_kernels"] = engine_rooflines(prof, updates)
+        out["propagation_kernels"] = add_random_access(engine_rooflines(prof, updates), prof,
+                                                       updates, N, out)
     if args.calibrate_copy_mib > 0:
         out["copy_calibration"] = copy_calibration(args.calibrate_copy_mib)
     print(json.dumps(out))
@@ -553,8 +593,10 @@ def main() -> int:
     ap.add_argument("--rates", type=float, nargs="*", default=[0.5, 1.0, 2.0, 4.0],
                     help="arrival rates as multiples of the plain path's capacity (serving)")
     ap.add_argument("--graphs", type=int, default=256, help="graphs of the batched workload")
-    ap.add_argument("--heavy-sweep", type=int, nargs="*", default=[-1, 65536],
+    ap.add_argument("--heavy-sweep", type=int, nargs="*", default=[-1, 65536, 16384, 4096],
                     help="batched: hand-back frontier sizes the autotuner tries")
+    ap.add_argument("--lanes-back", type=int, default=0,
+                    help="serving: hand-back lanes (0 = library default)")
     ap.add_argument("--heavy-frontier", type=int, default=0,
                     help="serving: frontier size at which a graph leaves the shared launch "
                          "(0 = library default, -1 = never)")
@@ -622,6 +664,8 @@ def main() -> int:
     ms_per_step = 1000.0 * wall / args.steps
     value = updates / (wall / args.steps)
     engines = engine_rooflines(prof, updates)
+    ra_out = {}
+    engines = add_random_access(engines, prof, updates, args.nodes, ra_out)
     dominant = max(engines, key=lambda k: engines[k]["kernel_ms_per_step"])
     dom = engines[dominant]
     achieved = dom["achieved_gbs"]
@@ -652,6 +696,7 @@ def main() -> int:
                      "avg_launch_us": dom["avg_launch_us"],
                      "algorithmic_bytes_per_launch": dom["algorithmic_bytes_per_launch"]},
         "propagation_kernels": engines,
+        "random_access": ra_out.get("random_access"),
         "tuning": tuning,
     }
     if not args.no_cpu:

@@ -216,7 +216,8 @@ typedef struct dlp_batch_stats {
 /* Tuning knobs of the device path (0 = built-in default). */
 typedef struct dlp_tuning {
   int propGroupLanes;   /* lanes per frontier vertex: 4, 8, 16, 32 or 64 */
-  int propBlocks;       /* blocks of the propagation launch              */
+  int propBlocks;       /* most blocks of a propagation launch (default
+                           8192; a launch sizes its grid to the frontier) */
   int chunkIterations;  /* iterations enqueued between host checks       */
   int localThreshold;   /* frontier size at or below which iterations run
                            in the LDS-resident single-work-group kernel;
@@ -366,10 +367,10 @@ typedef struct dlp_multi_tuning {
                            replay it (default), 2 = enqueue every time   */
   int heavyFrontier;    /* a graph whose frontier grows beyond this many
                            vertices leaves the shared launch and finishes
-                           its propagation on the whole GPU, after the
-                           launch (default -1 = never): lower latency for
-                           the other graphs of a step, but the handed-back
-                           graphs run one after another                  */
+                           its propagation with the single-graph engines
+                           on a stream of its own; -1 = never.  Default:
+                           never for dlp_multi_run_batch / run_next,
+                           4096 for dlp_multi_serve                      */
 } dlp_multi_tuning;
 
 /* One context per graph (arrays of numGraphs configs, graphs, true-label
@@ -408,13 +409,20 @@ typedef struct dlp_serve_config {
   int64_t hbmBudgetBytes; /* HBM working set of the admitted batches
                              (lists, G', sort scratch); 0 = unlimited  */
   int plain;              /* 1: one batch per step, single-graph path  */
+  int handBackLanes;      /* host threads (each with a stream) that
+                             finish handed-back graphs (multi tuning
+                             heavyFrontier > 0) beside later steps;
+                             0 = 8                                     */
 } dlp_serve_config;
 
 /* Arrival-driven continuous batching: requests (sorted by arrival) are
  * released at their arrival times; each step admits, in arrival order,
  * at most one batch per graph, up to maxSlots batches and the HBM budget,
  * and runs them together; completion[i] receives the time (seconds from
- * the start) at which request i's labels were ready. */
+ * the start) at which request i's labels were ready.  With hand-back on,
+ * a graph whose frontier outgrows heavyFrontier leaves its step and is
+ * finished by a hand-back lane while the next steps run; its tenant's
+ * next request waits for it. */
 int dlp_multi_serve(dlp_multi *m, const dlp_serve_request *reqs, int numReqs,
                     const dlp_serve_config *cfg, double *completion, int *steps);
 
@@ -429,6 +437,11 @@ int dlp_copy_to_host(void *dst, const void *src, size_t bytes);
 /* device -> device with the library's own copy kernel (16-byte vectors when
  * both pointers are 16-byte aligned); synchronous */
 int dlp_copy_device(void *dst, const void *src, size_t bytes);
+/* Random 4-byte accesses per second over an array of arrayInts ints
+ * (`accesses` per measurement): rates[0] returning atomicMax, rates[1]
+ * loads, rates[2] stores -- the ceilings of the gather- and atomic-bound
+ * propagation kernels (no counterpart in the reference). */
+int dlp_random_access_rates(int64_t arrayInts, int64_t accesses, double *rates);
 
 /* ====================================================================== */
 /* 7. IrLP: the full-recompute baseline (validation.h, main.cu:120-217)  */

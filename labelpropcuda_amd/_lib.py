@@ -137,7 +137,8 @@ class ServeRequest(C.Structure):
 
 
 class ServeConfig(C.Structure):
-    _fields_ = [("maxSlots", C.c_int), ("hbmBudgetBytes", C.c_int64), ("plain", C.c_int)]
+    _fields_ = [("maxSlots", C.c_int), ("hbmBudgetBytes", C.c_int64), ("plain", C.c_int),
+                ("handBackLanes", C.c_int)]
 
 
 class PropProfile(C.Structure):
@@ -244,6 +245,7 @@ SIGNATURES = {
                                              C.POINTER(C.c_int), C.c_int, C.c_uint, C.c_float,
                                              C.c_float]),
     "dlp_copy_device": (C.c_int, [_VP, _VP, C.c_size_t]),
+    "dlp_random_access_rates": (C.c_int, [C.c_int64, C.c_int64, C.POINTER(C.c_double)]),
     "dlp_write_graph_files": (C.c_int, [C.POINTER(GraphView), f32p, C.c_char_p]),
     "dlp_inputs_from_graph_files": (C.c_int, [C.POINTER(_VP), C.c_char_p, C.c_int, C.c_uint,
                                               C.c_float, C.c_float]),

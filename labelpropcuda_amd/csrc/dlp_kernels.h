@@ -31,7 +31,9 @@ struct PropArgs {
 };
 
 /* one propagation iteration k (reads qcount[k%3]); groupLanes in
- * {4,8,16,32,64} */
+ * {4,8,16,32,64}; blocks of kPropagateThreads threads */
+constexpr int kPropagateThreads = 512;
+constexpr int kMinPropagateBlocks = 512;
 int dlp_launch_propagate(const PropArgs &a, int k, int groupLanes, int blocks, hipStream_t s);
 
 /* global buffers of the LDS-resident propagation kernel */
@@ -148,7 +150,10 @@ struct PropTiming {
 
 struct PropLoopConfig {
   int groupLanes;     /* multi-block kernel lanes per vertex                 */
-  int blocks;         /* multi-block kernel grid                             */
+  int blocks;         /* multi-block kernel grid, at most: a chunk launches
+                         enough blocks for one lane group per vertex of twice
+                         the frontier it starts from, at least
+                         kMinPropagateBlocks                                 */
   int maxChunk;       /* multi-block iterations between host checks          */
   int localThreshold; /* frontier size at or below which the LDS kernel runs */
 };

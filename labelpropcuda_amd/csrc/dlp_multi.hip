@@ -22,7 +22,12 @@
  */
 #include <algorithm>
 #include <chrono>
+#include <condition_variable>
+#include <deque>
 #include <functional>
+#include <memory>
+#include <mutex>
+#include <string>
 #include <thread>
 #include <vector>
 
@@ -36,7 +41,10 @@ struct dlp_multi {
   int shape = 2;        /* LocalArgs::shape of the propagation launch */
   int backoff = 8;
   int thresholdPct = 100;
-  int heavy = -1;       /* hand-back frontier size; -1 = never           */
+  int heavy = 0;        /* hand-back frontier size; -1 = never, 0 = the
+                           default of the mode (lockstep batches: never;
+                           serving: kServeHandBack)                    */
+  int lanes = 8;        /* handed-back graphs finished side by side      */
   bool useGraphs = true;
   std::vector<GraphProp *> dProps; /* per batch: graphs that have it */
   std::vector<int> counts;
@@ -90,6 +98,16 @@ struct dlp_multi {
 
 namespace {
 
+/* hand-back frontier of serving steps when the tuning leaves it at 0: a
+ * graph whose frontier outgrows this leaves the step for a hand-back lane */
+constexpr int kServeHandBack = 4096;
+
+/* effective hand-back frontier (0 = none) of lockstep batches / serving */
+int heavy_lockstep(const dlp_multi *m) { return m->heavy > 0 ? m->heavy : 0; }
+int heavy_serving(const dlp_multi *m) {
+  return m->heavy > 0 ? m->heavy : (m->heavy == 0 ? kServeHandBack : 0);
+}
+
 /* per-graph descriptors of batch b's propagation launch */
 GraphProp graph_prop(const dlp_multi *m, int g);
 
@@ -128,7 +146,7 @@ GraphProp graph_prop(const dlp_multi *m, int gi) {
       m->thresholdPct < 0 ? 0 : (int)std::max<long long>(1, std::min<long long>(th, 1 << 30));
   if (c->N >= (1 << 30)) g.threshold = 0;  // outside-vertex encoding limit
   g.backoff = m->backoff;
-  g.heavy = m->heavy > 0 ? m->heavy : 0;
+  g.heavy = heavy_lockstep(m);
   g.exitIter = m->dExit + gi;
   return g;
 }
@@ -162,9 +180,9 @@ int enqueue_batch_post(dlp_multi *m, int b) {
 /* after a propagation launch over `graphs` (count n): the graphs it handed
  * back, read once the launch has finished (none when hand-back is off, in
  * which case nothing waits) */
-int handed_back(dlp_multi *m, const int *graphs, int n, std::vector<int> *handed) {
+int handed_back(dlp_multi *m, int heavy, const int *graphs, int n, std::vector<int> *handed) {
   handed->clear();
-  if (m->heavy <= 0 || n == 0) return DLP_OK;
+  if (heavy <= 0 || n == 0) return DLP_OK;
   DLP_HIP(hipMemcpyAsync(m->hExit, m->dExit, sizeof(int) * m->G, hipMemcpyDeviceToHost,
                          m->stream));
   DLP_HIP(hipStreamSynchronize(m->stream));
@@ -183,6 +201,156 @@ int handed_back(dlp_multi *m, const int *graphs, int n, std::vector<int> *handed
 
 /* the rest of a handed-back graph's propagation, on the shared stream */
 int continue_graph(dlp_multi *m, int g) { return dlp_ctx_run_propagation(m->ctx[g], m->hExit[g]); }
+
+/* Finishes handed-back graphs beside the serving loop's later steps: each
+ * of `workers` host threads takes handed-back graphs from a queue and runs
+ * them on a stream of its own (several run at once; the GPU overlaps their
+ * launches with each other and with the steps).  A handed-back graph is
+ * busy (not admitted again) until its propagation and batch end are done;
+ * its context uses the worker's stream meanwhile, and nothing else touches
+ * that context until it is released. */
+class HandBackLane {
+ public:
+  /* finishBatch: a task also runs the graph's batch end (serving); without
+   * it the caller runs the batch ends after drain() (lockstep batches) */
+  HandBackLane(dlp_multi *m, int workers, bool finishBatch,
+               std::function<void(int graph)> finished)
+      : m_(m), workers_(workers), finishBatch_(finishBatch), finished_(std::move(finished)),
+        busy_(m->G, 0) {}
+  ~HandBackLane() { stop(); }
+
+  int start() {
+    if (hipGetDevice(&device_) != hipSuccess) {
+      (void)hipGetLastError();
+      dlp_set_error_("dlp_multi_serve: hipGetDevice failed");
+      return DLP_ERR_HIP;
+    }
+    for (int i = 0; i < workers_; ++i) {
+      hipStream_t s = nullptr;
+      if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) {
+        (void)hipGetLastError();
+        dlp_set_error_("dlp_multi_serve: hipStreamCreate failed");
+        return DLP_ERR_HIP;
+      }
+      streams_.push_back(s);
+    }
+    for (int i = 0; i < workers_; ++i) threads_.emplace_back([this, i] { loop(streams_[i]); });
+    return DLP_OK;
+  }
+  /* graph g resumes at iteration k on the lane */
+  int push(int g, int k) {
+    std::lock_guard<std::mutex> lock(mu_);
+    busy_[g] = 1;
+    ++inFlight_;
+    queue_.push_back({g, k});
+    cv_.notify_all();
+    return DLP_OK;
+  }
+  bool busy(int g) {
+    std::lock_guard<std::mutex> lock(mu_);
+    return busy_[g] != 0;
+  }
+  int inFlight() {
+    std::lock_guard<std::mutex> lock(mu_);
+    return inFlight_;
+  }
+  int error() {
+    std::lock_guard<std::mutex> lock(mu_);
+    return rc_;
+  }
+  std::string message() {
+    std::lock_guard<std::mutex> lock(mu_);
+    return msg_;
+  }
+  /* waits until `deadline` (seconds on `now`) or until a graph is released */
+  template <class Now>
+  void wait(double deadline, Now now) {
+    std::unique_lock<std::mutex> lock(mu_);
+    const int seen = released_;
+    const double left = deadline - now();
+    if (left <= 0) return;
+    cv_.wait_for(lock, std::chrono::duration<double>(left),
+                 [&] { return released_ != seen || rc_ != DLP_OK; });
+  }
+  void drain() {
+    std::unique_lock<std::mutex> lock(mu_);
+    cv_.wait(lock, [&] { return inFlight_ == 0 || rc_ != DLP_OK; });
+  }
+  void stop() {
+    {
+      std::lock_guard<std::mutex> lock(mu_);
+      stop_ = true;
+      cv_.notify_all();
+    }
+    for (std::thread &t : threads_)
+      if (t.joinable()) t.join();
+    threads_.clear();
+    for (hipStream_t s : streams_) (void)hipStreamDestroy(s);
+    streams_.clear();
+  }
+
+ private:
+  struct Task {
+    int g, k;
+  };
+  void loop(hipStream_t stream) {
+    if (hipSetDevice(device_) != hipSuccess) {  // the device is per host thread
+      std::lock_guard<std::mutex> lock(mu_);
+      rc_ = DLP_ERR_HIP;
+      msg_ = "hipSetDevice failed";
+      cv_.notify_all();
+      return;
+    }
+    while (true) {
+      Task t;
+      {
+        std::unique_lock<std::mutex> lock(mu_);
+        cv_.wait(lock, [&] { return stop_ || !queue_.empty(); });
+        if (queue_.empty()) return;  // stop requested and nothing left
+        t = queue_.front();
+        queue_.pop_front();
+      }
+      dlp_context *c = m_->ctx[t.g];
+      hipStream_t own = c->stream;
+      c->stream = stream;
+      int rc = dlp_ctx_run_propagation(c, t.k);
+      if (rc == DLP_OK && finishBatch_)
+        rc = dlp_ctx_batch_post(c, c->nextBatch, dlp_ctx_make_batch(c, c->nextBatch));
+      if (rc == DLP_OK && hipStreamSynchronize(stream) != hipSuccess) {
+        (void)hipGetLastError();
+        dlp_set_error_("dlp_multi_serve: hand-back stream failed");
+        rc = DLP_ERR_HIP;
+      }
+      c->stream = own;
+      if (rc == DLP_OK) finished_(t.g);
+      const std::string msg = rc != DLP_OK ? dlp_last_error() : "";
+      std::lock_guard<std::mutex> lock(mu_);
+      if (rc != DLP_OK && rc_ == DLP_OK) {
+        rc_ = rc;
+        msg_ = msg;
+      }
+      busy_[t.g] = 0;
+      --inFlight_;
+      ++released_;
+      cv_.notify_all();
+    }
+  }
+  dlp_multi *m_;
+  int workers_;
+  bool finishBatch_;
+  std::function<void(int)> finished_;
+  std::vector<char> busy_;
+  std::deque<Task> queue_;
+  std::mutex mu_;
+  std::condition_variable cv_;
+  std::vector<std::thread> threads_;
+  std::vector<hipStream_t> streams_;
+  int device_ = 0;
+  std::string msg_;
+  int inFlight_ = 0, released_ = 0, rc_ = DLP_OK;
+  bool stop_ = false;
+};
+
 
 int enqueue_reset(dlp_multi *m) {
   for (dlp_context *c : m->ctx) {
@@ -418,9 +586,21 @@ extern "C" int dlp_multi_run_batch(dlp_multi *m, int b) {
   std::vector<int> members, handed;
   for (int g = 0; g < m->G; ++g)
     if (b < m->ctx[g]->B) members.push_back(g);
-  if ((rc = handed_back(m, members.data(), (int)members.size(), &handed)) != DLP_OK) return rc;
-  for (int g : handed)
-    if ((rc = continue_graph(m, g)) != DLP_OK) return rc;
+  if ((rc = handed_back(m, heavy_lockstep(m), members.data(), (int)members.size(), &handed)) !=
+      DLP_OK)
+    return rc;
+  if (handed.size() == 1) {
+    if ((rc = continue_graph(m, handed[0])) != DLP_OK) return rc;
+  } else if (!handed.empty()) {  // several: side by side, each on a stream of its own
+    HandBackLane lane(m, std::min<int>((int)handed.size(), m->lanes), false, [](int) {});
+    if ((rc = lane.start()) != DLP_OK) return rc;
+    for (int g : handed) lane.push(g, m->hExit[g]);
+    lane.drain();
+    if ((rc = lane.error()) != DLP_OK) {
+      dlp_set_errorf_("graph continuation: %s", lane.message().c_str());
+      return rc;
+    }
+  }
   if (m->useGraphs) {
     DLP_HIP(hipGraphLaunch(m->execPost[b], m->stream));
     if (!justCaptured)
@@ -519,9 +699,11 @@ namespace {
 /* the next batch of `graphs` (validated) together.  With `done` set, it is
  * called with the stream idle for the graphs whose batch has completed:
  * first all that the propagation launch finished, then each handed-back
- * graph on its own. */
-int run_next_impl(dlp_multi *m, const int *graphs, int count,
-                  const std::function<void(const std::vector<int> &)> *done) {
+ * graph on its own.  With `handOff` set, a handed-back graph is passed to
+ * it (graph, resume iteration) instead of being finished here. */
+int run_next_impl(dlp_multi *m, const int *graphs, int count, int heavy,
+                  const std::function<void(const std::vector<int> &)> *done,
+                  const std::function<int(int, int)> *handOff = nullptr) {
   DLP_HIP(hipStreamSynchronize(m->stream));  // the staging buffer is free
   int rc;
   for (int i = 0; i < count; ++i) {
@@ -530,6 +712,7 @@ int run_next_impl(dlp_multi *m, const int *graphs, int count,
     int tag = 0;
     if ((rc = dlp_ctx_batch_pre(c, b, dlp_ctx_make_batch(c, b), &tag)) != DLP_OK) return rc;
     m->hStep[i] = m->props[graphs[i]];
+    m->hStep[i].heavy = heavy;
   }
   DLP_HIP(hipMemcpyAsync(m->dStep, m->hStep, sizeof(GraphProp) * count, hipMemcpyHostToDevice,
                          m->stream));
@@ -537,7 +720,7 @@ int run_next_impl(dlp_multi *m, const int *graphs, int count,
       DLP_OK)
     return rc;
   std::vector<int> handed, finished;
-  if ((rc = handed_back(m, graphs, count, &handed)) != DLP_OK) return rc;
+  if ((rc = handed_back(m, heavy, graphs, count, &handed)) != DLP_OK) return rc;
   auto post = [&](int g) {
     dlp_context *c = m->ctx[g];
     return dlp_ctx_batch_post(c, c->nextBatch, dlp_ctx_make_batch(c, c->nextBatch));
@@ -552,6 +735,10 @@ int run_next_impl(dlp_multi *m, const int *graphs, int count,
     (*done)(finished);
   }
   for (int g : handed) {
+    if (handOff) {
+      if ((rc = (*handOff)(g, m->hExit[g])) != DLP_OK) return rc;
+      continue;
+    }
     if ((rc = continue_graph(m, g)) != DLP_OK || (rc = post(g)) != DLP_OK) return rc;
     if (done) {
       DLP_HIP(hipStreamSynchronize(m->stream));
@@ -582,7 +769,7 @@ extern "C" int dlp_multi_run_next(dlp_multi *m, const int *graphs, int count) {
       return DLP_ERR_STATE;
     }
   }
-  return run_next_impl(m, graphs, count, nullptr);
+  return run_next_impl(m, graphs, count, heavy_lockstep(m), nullptr);
 }
 
 namespace {
@@ -628,9 +815,31 @@ extern "C" int dlp_multi_serve(dlp_multi *m, const dlp_serve_request *reqs, int 
   int nextArrival = 0, nSteps = 0, rc;
   std::vector<char> blocked(m->G, 0);
   std::vector<int> admitted, graphs;
+  /* handed-back graphs finish beside the following steps */
+  std::vector<int> laneReq(m->G, -1);  // request a graph on the lane serves
+  std::unique_ptr<HandBackLane> lane;
+  const int heavy = cfg->plain ? 0 : heavy_serving(m);
+  if (heavy > 0) {
+    const int workers = cfg->handBackLanes > 0 ? cfg->handBackLanes : 8;
+    lane.reset(new HandBackLane(m, workers, true,
+                                [&](int g) { completion[laneReq[g]] = now(); }));
+    if ((rc = lane->start()) != DLP_OK) return rc;
+  }
+  const std::function<int(int, int)> handOff = [&](int g, int k) {
+    for (size_t i = 0; i < graphs.size(); ++i)
+      if (graphs[i] == g) laneReq[g] = admitted[i];
+    return lane->push(g, k);
+  };
+  auto laneFailed = [&]() {
+    rc = lane ? lane->error() : DLP_OK;
+    if (rc != DLP_OK) dlp_set_errorf_("dlp_multi_serve: hand-back lane: %s", lane->message().c_str());
+    return rc != DLP_OK;
+  };
   while (nextArrival < numReqs || !pending.empty()) {
+    if (laneFailed()) return rc;
     const double t = now();
     while (nextArrival < numReqs && reqs[nextArrival].arrival <= t) pending.push_back(nextArrival++);
+    const double nextAt = nextArrival < numReqs ? reqs[nextArrival].arrival : t + 1.0;
     if (pending.empty()) {
       std::this_thread::sleep_for(std::chrono::duration<double>(reqs[nextArrival].arrival - t));
       continue;
@@ -646,6 +855,7 @@ extern "C" int dlp_multi_serve(dlp_multi *m, const dlp_serve_request *reqs, int 
       const int g = reqs[idx].graph;
       if (blocked[g]) continue;
       blocked[g] = 1;  // later requests of g wait for this one
+      if (lane && lane->busy(g)) continue;
       if ((int)admitted.size() >= slots) continue;
       const int64_t w = batch_work_bytes(m->ctx[g], m->ctx[g]->nextBatch);
       if (!admitted.empty() && cfg->hbmBudgetBytes > 0 && used + w > cfg->hbmBudgetBytes)
@@ -653,6 +863,10 @@ extern "C" int dlp_multi_serve(dlp_multi *m, const dlp_serve_request *reqs, int 
       used += w;
       admitted.push_back(idx);
       graphs.push_back(g);
+    }
+    if (admitted.empty()) {  // every pending graph is busy on the lane
+      lane->wait(nextAt, now);
+      continue;
     }
     if (cfg->plain) {
       dlp_context *c = m->ctx[graphs[0]];
@@ -669,7 +883,9 @@ extern "C" int dlp_multi_serve(dlp_multi *m, const dlp_serve_request *reqs, int 
               if (std::find(done.begin(), done.end(), graphs[i]) != done.end())
                 completion[admitted[i]] = t;
           };
-      if ((rc = run_next_impl(m, graphs.data(), (int)graphs.size(), &stamp)) != DLP_OK) return rc;
+      if ((rc = run_next_impl(m, graphs.data(), (int)graphs.size(), heavy, &stamp,
+                              lane ? &handOff : nullptr)) != DLP_OK)
+        return rc;
     }
     ++nSteps;
     std::vector<int> rest;
@@ -683,6 +899,10 @@ extern "C" int dlp_multi_serve(dlp_multi *m, const dlp_serve_request *reqs, int 
       rest.push_back(idx);
     }
     pending.swap(rest);
+  }
+  if (lane) {
+    lane->drain();
+    if (laneFailed()) return rc;
   }
   if (steps) *steps = nSteps;
   return DLP_OK;

@@ -111,12 +111,20 @@ int dlp_run_propagation(const PropArgs &a, const LocalArgs *l, int maxIter,
     }
     chunk = std::min(chunk, maxChunk);
     const int m = std::min(chunk, maxIter - k);
+    /* grid: one lane group per vertex of twice the chunk's first frontier
+     * (frontiers grow within a chunk), at least kMinPropagateBlocks (a small
+     * frontier spread over many compute units hides latency better than
+     * the same waves packed onto a few) and at most cfg.blocks */
+    const long long want =
+        (2LL * n * cfg.groupLanes + kPropagateThreads - 1) / kPropagateThreads;
+    const int blocks = (int)std::max<long long>(
+        1, std::min<long long>(cfg.blocks, std::max<long long>(want, kMinPropagateBlocks)));
     PropTiming *perLaunch = (timing && !timing->perChunk) ? timing : nullptr;
     if (timing && timing->perChunk && (rc = begin_timed(timing, s, false, m)) != DLP_OK)
       return rc;
     for (int j = 0; j < m; ++j) {
       if ((rc = begin_timed(perLaunch, s, false, 1)) != DLP_OK) return rc;
-      if ((rc = dlp_launch_propagate(a, k + j, cfg.groupLanes, cfg.blocks, s)) != DLP_OK)
+      if ((rc = dlp_launch_propagate(a, k + j, cfg.groupLanes, blocks, s)) != DLP_OK)
         return rc;
       if ((rc = end_timed(perLaunch, s)) != DLP_OK) return rc;
     }

@@ -70,16 +70,68 @@ __device__ __forceinline__ float irlp_label(int gt, bool live, float W, float S,
 }
 
 constexpr int kIrlpLaneRowMax = 16;
+constexpr int kIrlpStage = 1024; /* entries of a wave's rows staged in LDS */
+
+/* row position p of a staged row: weight and neighbour label copied to LDS,
+ * both 0 for an entry that adds nothing (the product is then +0.0 too) */
+struct StagedPos {
+  const float *w;
+  const float *f;
+  int off;
+  __device__ void operator()(int p, float &ww, float &ss) const {
+    ww = w[off + p];
+    ss = ww * f[off + p];
+  }
+};
+
+/* the labels of a wave's 64 consecutive vertices (lane = vertex): rows up to
+ * 16 entries by their own lane, longer rows by the whole wave one at a time;
+ * `at(b)` is the position functor of a row starting at entry b */
+template <class At>
+__device__ __forceinline__ void irlp_wave_rows(const At &at, long long base, int lane, bool act,
+                                               int gt, bool live, int beg, int len,
+                                               const float *Fold, float *Fnew, float delta,
+                                               bool &changed) {
+  const long long u = base + lane;
+  const bool laneRow = live && len <= kIrlpLaneRowMax;
+  const int lenS = laneRow ? len : 0;
+  float W, S;
+  if (!__any(lenS > 4))
+    dlp::lane_row_sums<4>(lenS, at(beg), W, S);
+  else if (!__any(lenS > 8))
+    dlp::lane_row_sums<8>(lenS, at(beg), W, S);
+  else
+    dlp::lane_row_sums<kIrlpLaneRowMax>(lenS, at(beg), W, S);
+  if (act && (laneRow || !live)) Fnew[u] = irlp_label(gt, live, W, S, Fold[u], delta, changed);
+  unsigned long long longMask = dlp_ballot(live && !laneRow);
+  while (longMask) {  // wave-uniform
+    const int src = __ffsll((long long)longMask) - 1;
+    longMask &= longMask - 1;
+    const int lbeg = __shfl(beg, src, 64), llen = __shfl(len, src, 64);
+    float LW, LS;
+    dlp::group_row_sums<64>(lane, llen, at(lbeg), LW, LS);
+    if (lane == 0) {
+      const long long lu = base + src;
+      Fnew[lu] = irlp_label(-1, true, LW, LS, Fold[lu], delta, changed);
+    }
+  }
+}
 
 /* one lane per vertex, 64 consecutive vertices per wave; iteration `iter`
- * as word j of the chunk */
+ * as word j of the chunk.  When the wave's rows hold at most kIrlpStage
+ * entries, their weights and neighbour labels are first copied to LDS with
+ * coalesced loads (one label gather per entry, all independent), and the
+ * rows are summed from there. */
 __global__ __launch_bounds__(kIrlpThreads) void k_irlp_lanes(IrlpArgs a, int iter, int j) {
+  __shared__ float stW[kIrlpThreads / 64][kIrlpStage];
+  __shared__ float stF[kIrlpThreads / 64][kIrlpStage];
   if (a.flags[j] == 0) return;  // converged before this iteration (grid-uniform)
   const float *Fold = (iter & 1) ? a.lab1 : a.lab0;
   float *Fnew = (iter & 1) ? a.lab0 : a.lab1;
   const int *col = a.colMasked ? a.colMasked : a.colInd;
   const int *nbStatus = a.colMasked ? nullptr : a.status;
-  const int lane = threadIdx.x & 63;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  float *sw = stW[wv], *sf = stF[wv];
   const long long gwave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const long long nwaves = ((long long)gridDim.x * blockDim.x) >> 6;
   const long long N = a.numNodes;
@@ -97,30 +149,22 @@ __global__ __launch_bounds__(kIrlpThreads) void k_irlp_lanes(IrlpArgs a, int ite
       beg = a.rowPtr[u];
       len = a.rowPtr[u + 1] - beg;
     }
-    const bool laneRow = live && len <= kIrlpLaneRowMax;
-    const int lenS = laneRow ? len : 0;
-    const dlp::GlobalPos pos{col, a.values, Fold, nbStatus, beg};
-    float W, S;
-    if (!__any(lenS > 4))
-      dlp::lane_row_sums<4>(lenS, pos, W, S);
-    else if (!__any(lenS > 8))
-      dlp::lane_row_sums<8>(lenS, pos, W, S);
-    else
-      dlp::lane_row_sums<kIrlpLaneRowMax>(lenS, pos, W, S);
-    if (act && (laneRow || !live)) Fnew[u] = irlp_label(gt, live, W, S, Fold[u], a.delta, changed);
-    /* longer rows: the wave takes them one by one */
-    unsigned long long longMask = dlp_ballot(live && !laneRow);
-    while (longMask) {  // wave-uniform
-      const int src = __ffsll((long long)longMask) - 1;
-      longMask &= longMask - 1;
-      const int lbeg = __shfl(beg, src, 64), llen = __shfl(len, src, 64);
-      float LW, LS;
-      dlp::group_row_sums<64>(lane, llen, dlp::GlobalPos{col, a.values, Fold, nbStatus, lbeg},
-                              LW, LS);
-      if (lane == 0) {
-        const long long lu = base + src;
-        Fnew[lu] = irlp_label(-1, true, LW, LS, Fold[lu], a.delta, changed);
+    const int e0 = a.rowPtr[base];
+    const int e1 = a.rowPtr[base + 64 < N ? base + 64 : N];
+    if (e1 - e0 <= kIrlpStage) {  // wave-uniform
+      for (int i = lane; i < e1 - e0; i += 64) {
+        const int v = col[e0 + i];
+        const bool ok = v >= 0 && !(nbStatus && nbStatus[v] == DLP_VERTEX_DELETED);
+        sw[i] = ok ? a.values[e0 + i] : 0.0f;
+        sf[i] = ok ? Fold[v] : 0.0f;
       }
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");  // stage written before read
+      irlp_wave_rows([&](int b) { return StagedPos{sw, sf, b - e0}; }, base, lane, act, gt, live,
+                     beg, len, Fold, Fnew, a.delta, changed);
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");  // read before the next stage
+    } else {
+      irlp_wave_rows([&](int b) { return dlp::GlobalPos{col, a.values, Fold, nbStatus, b}; },
+                     base, lane, act, gt, live, beg, len, Fold, Fnew, a.delta, changed);
     }
   }
   if (__any(changed) && lane == 0) a.flags[j + 1] = 1;

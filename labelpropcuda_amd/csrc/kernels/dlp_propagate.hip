@@ -38,7 +38,7 @@ namespace {
 /* ------------------------------------------------------------------------ */
 using dlp::GlobalPos;
 
-constexpr int kPT = 512;       /* threads of the multi-block kernel          */
+constexpr int kPT = kPropagateThreads; /* threads of the multi-block kernel */
 constexpr int kStageQ = 4096;  /* per-block staged next-frontier appends     */
 constexpr int kStageT = 2048;  /* per-block staged touched-list appends      */
 
@@ -74,6 +74,25 @@ __device__ __forceinline__ void stage_flush(const int *stage, int count, int *gC
   const int b = *baseSlot;
   for (int i = threadIdx.x; i < nb; i += blockDim.x) gList[b + i] = stage[i];
 }
+
+/* Claims vertex v (stamp word `stamp`) for the frontier of the iteration
+ * whose stamp is stampNext and returns the stamp it held before; the caller
+ * appends v when that is below stampNext.  With `filter`, an agent-scope
+ * load first skips the atomic when v is ineligible (INT_MAX) or already
+ * claimed in this iteration: on large frontiers a vertex is usually marked
+ * by several neighbours and returning atomics are the throughput limit
+ * (about half the rate of random loads); on small ones the extra round trip
+ * only adds latency. */
+__device__ __forceinline__ int claim_stamp(int *stamp, int stampNext, bool filter) {
+  if (filter) {
+    const int cur = __hip_atomic_load(stamp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (cur >= stampNext) return cur;
+  }
+  return atomicMax(stamp, stampNext);
+}
+
+/* frontier size from which k_propagate filters its claims */
+constexpr int kClaimFilterMin = 16384;
 
 template <int G>
 __global__ __launch_bounds__(kPT) void k_propagate(PropArgs a, int k) {
@@ -146,7 +165,8 @@ __global__ __launch_bounds__(kPT) void k_propagate(PropArgs a, int k) {
         v = u;
         pred = true;
       }
-      const int old = pred ? atomicMax(&a.iterStamp[v], stampNext) : DLP_INT_MAX;
+      const int old = pred ? claim_stamp(&a.iterStamp[v], stampNext, n >= kClaimFilterMin)
+                           : DLP_INT_MAX;
       const bool app = pred && old < stampNext;
       profA += app ? 1u : 0u;
       stage_append<kStageQ>(stageQ, &stageCount[0], outCount, out, app, v);

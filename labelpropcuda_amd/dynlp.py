@@ -514,13 +514,13 @@ class Multi:
         check(self._lib.dlp_multi_run_next(self._h, _ptr(arr, C.c_int), len(arr)))
 
     def serve(self, requests, maxSlots: int = 64, hbmBudgetBytes: int = 0,
-              plain: bool = False):
+              plain: bool = False, handBackLanes: int = 0):
         """requests: sequence of (graph, arrival seconds), sorted by arrival.
         Returns (completion seconds per request, steps)."""
         n = len(requests)
         reqs = (_lib.ServeRequest * max(n, 1))(*[_lib.ServeRequest(int(g), float(t))
                                                   for g, t in requests])
-        cfg = _lib.ServeConfig(maxSlots, hbmBudgetBytes, 1 if plain else 0)
+        cfg = _lib.ServeConfig(maxSlots, hbmBudgetBytes, 1 if plain else 0, handBackLanes)
         done = np.zeros(max(n, 1), np.float64)
         steps = C.c_int()
         check(self._lib.dlp_multi_serve(self._h, reqs, n, C.byref(cfg),

@@ -44,17 +44,19 @@ def requests(tenants, rng, spread):
     return sorted(reqs, key=lambda r: r[1])
 
 
-@pytest.mark.parametrize("slots,budget,plain,spread,heavy", [
-    (64, 0, False, 0.0, 0),        # everything at once: maximal mixing
-    (3, 0, False, 0.05, 0),        # few slots, staggered arrivals
-    (64, 4096, False, 0.0, 0),     # tiny HBM budget: one batch per step
-    (1, 0, True, 0.0, 0),          # plain: one batch at a time, single-graph path
-    (64, 0, False, 0.0, 1),        # every step hands its graphs back
-    (4, 0, False, 0.02, 200),      # some graphs handed back, staggered
-    (64, 0, False, 0.0, -1),       # no hand-back
+@pytest.mark.parametrize("slots,budget,plain,spread,heavy,lanes", [
+    (64, 0, False, 0.0, 0, 0),        # everything at once: maximal mixing
+    (3, 0, False, 0.05, 0, 0),        # few slots, staggered arrivals
+    (64, 4096, False, 0.0, 0, 0),     # tiny HBM budget: one batch per step
+    (1, 0, True, 0.0, 0, 0),          # plain: one batch at a time, single-graph path
+    (64, 0, False, 0.0, 1, 0),        # every step hands its graphs back (default lanes)
+    (64, 0, False, 0.0, 1, 1),        # ... to a single lane
+    (4, 0, False, 0.02, 200, 0),      # some graphs handed back, staggered
+    (8, 0, False, 0.01, 100, 5),      # five lanes, staggered
+    (64, 0, False, 0.0, -1, 0),       # no hand-back
 ])
 def test_serving_same_alone_and_in_mix(gpu_available, tenants, slots, budget, plain, spread,
-                                       heavy):
+                                       heavy, lanes):
     ref = alone(tenants, 3000)
     cfgs = [L.make_config(t, maxIterations=3000) for t in tenants]
     rng = np.random.default_rng(11)
@@ -63,7 +65,8 @@ def test_serving_same_alone_and_in_mix(gpu_available, tenants, slots, budget, pl
         m.reset()
         m.run_next(list(range(len(tenants))))  # seed batches: tenants become resident
         reqs = requests(tenants, rng, spread)
-        done, steps = m.serve(reqs, maxSlots=slots, hbmBudgetBytes=budget, plain=plain)
+        done, steps = m.serve(reqs, maxSlots=slots, hbmBudgetBytes=budget, plain=plain,
+                              handBackLanes=lanes)
         m.synchronize()
         assert np.all(done >= np.array([r[1] for r in reqs]))
         if plain or budget:
