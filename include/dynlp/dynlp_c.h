@@ -216,8 +216,9 @@ typedef struct dlp_batch_stats {
 /* Tuning knobs of the device path (0 = built-in default). */
 typedef struct dlp_tuning {
   int propGroupLanes;   /* lanes per frontier vertex: 4, 8, 16, 32 or 64 */
-  int propBlocks;       /* most blocks of a propagation launch (default
-                           8192; a launch sizes its grid to the frontier) */
+  int propBlocks;       /* most blocks of a propagation launch (a launch
+                           sizes its grid to the frontier); 0 = by graph
+                           size: N / 4096 within [512, 8192]            */
   int chunkIterations;  /* iterations enqueued between host checks       */
   int localThreshold;   /* frontier size at or below which iterations run
                            in the LDS-resident single-work-group kernel;

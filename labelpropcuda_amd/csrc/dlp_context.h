@@ -31,7 +31,8 @@ struct dlp_context {
   int *iterSizes = nullptr;
   int iterSizesCap = 0;
   DevCounters *pinCtr = nullptr;
-  dlp_tuning tuning{8, 8192, 8, 0, 0, 1}; /* local threshold and LDS set at creation */
+  dlp_tuning tuning{8, 0, 8, 0, 0, 1}; /* propBlocks 0 = by graph size; local
+                                          threshold and LDS set at creation */
   LocalArgs local{};
   int localMaxBytes = 0;
   int nextBatch = 0;

@@ -34,6 +34,17 @@ struct PropArgs {
  * {4,8,16,32,64}; blocks of kPropagateThreads threads */
 constexpr int kPropagateThreads = 512;
 constexpr int kMinPropagateBlocks = 512;
+/* Default grid cap of the multi-block kernel for a graph of N vertices:
+ * while the per-vertex arrays stay in the caches (up to a few million
+ * vertices) launches are short and more blocks only add dispatch and
+ * per-block flush work (C2, 500,000 vertices: 566 ms per step at 512 blocks,
+ * 603 ms at 8,192); once the label and stamp gathers go to HBM, more waves
+ * in flight hide their latency (C3, 50 M vertices: 37.0 s at 512, 23.0 s at
+ * 8,192). */
+inline int dlp_auto_prop_blocks(long long numNodes) {
+  const long long b = numNodes / 4096;
+  return (int)(b < kMinPropagateBlocks ? kMinPropagateBlocks : (b > 8192 ? 8192 : b));
+}
 int dlp_launch_propagate(const PropArgs &a, int k, int groupLanes, int blocks, hipStream_t s);
 
 /* global buffers of the LDS-resident propagation kernel */

@@ -203,7 +203,7 @@ namespace {
 int run_propagation(dlp_context *c, const PropArgs &a, int maxIter, int startK = 0) {
   PropLoopConfig cfg;
   cfg.groupLanes = c->tuning.propGroupLanes;
-  cfg.blocks = c->tuning.propBlocks;
+  cfg.blocks = c->tuning.propBlocks > 0 ? c->tuning.propBlocks : dlp_auto_prop_blocks(c->N);
   cfg.maxChunk = c->tuning.chunkIterations;
   cfg.localThreshold = c->tuning.localThreshold;
   /* the local kernel encodes outside vertices as -2 - (2 * id + 1) */

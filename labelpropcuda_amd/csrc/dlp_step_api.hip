@@ -353,7 +353,7 @@ extern "C" int dlp_iterative_propagation(const int *rowPtr, const int *colInd,
   l.ovf = ovf;
   PropLoopConfig cfg;
   cfg.groupLanes = 8;
-  cfg.blocks = 8192;
+  cfg.blocks = dlp_auto_prop_blocks(numNodes);
   cfg.maxChunk = 16;
   {
     int nnz = 0;
