@@ -222,14 +222,14 @@ class HandBackLane {
   int start() {
     if (hipGetDevice(&device_) != hipSuccess) {
       (void)hipGetLastError();
-      dlp_set_error_("dlp_multi_serve: hipGetDevice failed");
+      dlp_set_error_("hand-back lane: hipGetDevice failed");
       return DLP_ERR_HIP;
     }
     for (int i = 0; i < workers_; ++i) {
       hipStream_t s = nullptr;
       if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) {
         (void)hipGetLastError();
-        dlp_set_error_("dlp_multi_serve: hipStreamCreate failed");
+        dlp_set_error_("hand-back lane: hipStreamCreate failed");
         return DLP_ERR_HIP;
       }
       streams_.push_back(s);
@@ -318,7 +318,7 @@ class HandBackLane {
         rc = dlp_ctx_batch_post(c, c->nextBatch, dlp_ctx_make_batch(c, c->nextBatch));
       if (rc == DLP_OK && hipStreamSynchronize(stream) != hipSuccess) {
         (void)hipGetLastError();
-        dlp_set_error_("dlp_multi_serve: hand-back stream failed");
+        dlp_set_error_("hand-back lane: stream synchronisation failed");
         rc = DLP_ERR_HIP;
       }
       c->stream = own;
