@@ -12,6 +12,9 @@
 /* propagation (dlp_propagate.hip)                                          */
 /* ------------------------------------------------------------------------ */
 struct PropArgs {
+  const int2 *rowInfo; /* nullable: {row start, row length} per vertex; the
+                          multi-block and work-group HBM paths read it in
+                          place of rowBeg / rowLen                           */
   const int *rowBeg;  /* row start per vertex                                  */
   const int *rowEnd;  /* standard CSR: rowPtr + 1 (used when rowLen == null)   */
   const int *rowLen;  /* slack layout: active row length                       */
@@ -188,6 +191,7 @@ struct DState {
   const float *fVal;
   /* active CSR, slack layout: row u occupies [fRow[u], fRow[u] + aLen[u]) */
   int *aLen;
+  int2 *aRow;     /* {fRow[u], aLen[u]}: one load per frontier vertex      */
   int *aCol;
   float *aVal;
   /* vertex state */

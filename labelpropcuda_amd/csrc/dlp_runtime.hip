@@ -379,6 +379,7 @@ extern "C" int dlp_context_create(dlp_context **out, const dlp_config *cfg,
   TRY(c->upload(&c->dEW, bv->newEdgeWeight + bv->edgeOffsets[0], c->eOff[B]));
 
   TRY(c->alloc(&d.aLen, N));
+  TRY(c->alloc(&d.aRow, N));
   TRY(c->alloc(&d.aCol, E));
   TRY(c->alloc(&d.aVal, E));
   TRY(c->alloc(&d.hStatus, N));
@@ -643,6 +644,7 @@ int dlp_ctx_run_propagation(dlp_context *c, int startK) {
 PropArgs dlp_ctx_prop_args(const dlp_context *c) {
   const DState &d = c->d;
   PropArgs a{};
+  a.rowInfo = d.aRow;
   a.rowBeg = d.fRow;
   a.rowEnd = nullptr;
   a.rowLen = d.aLen;

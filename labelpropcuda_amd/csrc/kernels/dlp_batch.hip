@@ -118,6 +118,7 @@ __global__ void k_reset(DState d) {
     d.markStamp[v] = 0;
     d.compId[v] = -1;
     d.aLen[v] = 0;
+    d.aRow[v] = make_int2(d.fRow[v], 0);
   }
   for (long long w = i; w < d.numWords; w += stride) d.newBits[w] = 0u;
   if (i == 0) {
@@ -249,7 +250,10 @@ __global__ __launch_bounds__(kT) void k_csr_update(DState d, int markTag) {
       dlp_mark_candidate(d.markStamp, markTag, d.cand, &d.ctr->numCand, v, keep && q1);
       written += __popc(gm);
     }
-    if (act && gl == 0) d.aLen[u] = written;
+    if (act && gl == 0) {
+      d.aLen[u] = written;
+      d.aRow[u] = make_int2(fb, written);
+    }
   }
 }
 

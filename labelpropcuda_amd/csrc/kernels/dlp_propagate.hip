@@ -136,8 +136,14 @@ __global__ __launch_bounds__(kPT) void k_propagate(PropArgs a, int k) {
     int u = 0, beg = 0, len = 0;
     if (act) {
       u = in[item];
-      beg = a.rowBeg[u];
-      len = a.rowLen ? a.rowLen[u] : a.rowEnd[u] - beg;
+      if (a.rowInfo) {
+        const int2 r = a.rowInfo[u];
+        beg = r.x;
+        len = r.y;
+      } else {
+        beg = a.rowBeg[u];
+        len = a.rowLen ? a.rowLen[u] : a.rowEnd[u] - beg;
+      }
     }
     float W, S;
     dlp::group_row_sums<G>(gl, len, GlobalPos{a.colInd, a.values, Fold, a.status, beg}, W, S);
@@ -926,8 +932,15 @@ __device__ __forceinline__ void wg_iterations(const PropArgs &a, int &k, int max
       const int item = base + tid;
       const bool act = item < n;
       const int u = act ? in[item] : 0;
-      const int beg = act ? a.rowBeg[u] : 0;
-      const int len = act ? (a.rowLen ? a.rowLen[u] : a.rowEnd[u] - beg) : 0;
+      int beg = 0, len = 0;
+      if (act && a.rowInfo) {
+        const int2 r = a.rowInfo[u];
+        beg = r.x;
+        len = r.y;
+      } else if (act) {
+        beg = a.rowBeg[u];
+        len = a.rowLen ? a.rowLen[u] : a.rowEnd[u] - beg;
+      }
       const bool laneRow = act && len <= kWgLaneRowMax;
       const int lenS = laneRow ? len : 0;
       if (!__any(lenS > 4))
