@@ -219,7 +219,8 @@ typedef struct dlp_tuning {
   int propBlocks;       /* most blocks of a propagation launch (a launch
                            sizes its grid to the frontier); 0 = by graph
                            size: N / 4096 within [512, 8192]            */
-  int chunkIterations;  /* iterations enqueued between host checks       */
+  int chunkIterations;  /* most multi-block iterations enqueued between
+                           host checks (chunks double up to it; 1024)    */
   int localThreshold;   /* frontier size at or below which iterations run
                            in the LDS-resident single-work-group kernel;
                            < 0 disables it                               */

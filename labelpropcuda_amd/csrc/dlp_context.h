@@ -31,8 +31,12 @@ struct dlp_context {
   int *iterSizes = nullptr;
   int iterSizesCap = 0;
   DevCounters *pinCtr = nullptr;
-  dlp_tuning tuning{8, 0, 8, 0, 0, 1}; /* propBlocks 0 = by graph size; local
-                                          threshold and LDS set at creation */
+  /* propBlocks 0 = by graph size; chunks of up to 1024 multi-block
+   * launches (C2: 8 -> 471.9, 64 -> 498.9, 256 -> 523.5, 1024 -> 539.8 M
+   * updates/s: fewer host round trips, and after a working set outgrew LDS
+   * the LDS kernel is retried only after a chunk); local threshold and LDS
+   * set at creation */
+  dlp_tuning tuning{8, 0, 1024, 0, 0, 1};
   LocalArgs local{};
   int localMaxBytes = 0;
   int nextBatch = 0;

@@ -354,7 +354,7 @@ extern "C" int dlp_iterative_propagation(const int *rowPtr, const int *colInd,
   PropLoopConfig cfg;
   cfg.groupLanes = 8;
   cfg.blocks = dlp_auto_prop_blocks(numNodes);
-  cfg.maxChunk = 16;
+  cfg.maxChunk = 1024;
   {
     int nnz = 0;
     DLP_HIP(hipMemcpyAsync(&nnz, rowPtr + numNodes, sizeof(int), hipMemcpyDeviceToHost, s));
