@@ -372,7 +372,7 @@ typedef struct dlp_multi_tuning {
                            its propagation with the single-graph engines
                            on a stream of its own; -1 = never.  Default:
                            never for dlp_multi_run_batch / run_next,
-                           4096 for dlp_multi_serve                      */
+                           8192 for dlp_multi_serve                      */
 } dlp_multi_tuning;
 
 /* One context per graph (arrays of numGraphs configs, graphs, true-label
