@@ -229,6 +229,10 @@ typedef struct dlp_tuning {
   int localShape;       /* work-group shape of the local kernel, 1-based
                            (threads / records per padded row): 1 = 1024/8,
                            2 = 1024/16, 3 = 512/8, 4 = 256/8, 5 = 1024/4 */
+  int localOwnerMode;   /* owner mode of the local kernel (one thread per
+                           core vertex, row in registers, frontier as
+                           flags) when the core fits: 1 on (default),
+                           -1 off (queue iterations only)                */
 } dlp_tuning;
 
 /* Uploads the full graph, true labels and every batch, allocates all device
@@ -303,6 +307,7 @@ typedef struct dlp_prop_profile {
   int64_t localEdges;
   int64_t localChangedEdges;
   int64_t localAppended;
+  int64_t localOwnerUpdates; /* of localUpdates: processed in owner mode   */
 } dlp_prop_profile;
 
 /* Sets the profiling level and clears the accumulated profile:

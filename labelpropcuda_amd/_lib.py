@@ -100,6 +100,7 @@ class Tuning(C.Structure):
         ("localThreshold", C.c_int),
         ("localLdsBytes", C.c_int),
         ("localShape", C.c_int),
+        ("localOwnerMode", C.c_int),
     ]
 
 
@@ -154,6 +155,7 @@ class PropProfile(C.Structure):
         ("localEdges", C.c_int64),
         ("localChangedEdges", C.c_int64),
         ("localAppended", C.c_int64),
+        ("localOwnerUpdates", C.c_int64),
     ]
 
 

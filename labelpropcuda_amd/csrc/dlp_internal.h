@@ -66,6 +66,7 @@ struct DevCounters {
   unsigned long long profLocalEdges;
   unsigned long long profLocalChangedEdges;
   unsigned long long profLocalAppended;
+  unsigned long long profLocalOwnerUpdates; /* of profLocalUpdates: owner mode */
 };
 
 /* exit reasons of the LDS-resident propagation kernel */

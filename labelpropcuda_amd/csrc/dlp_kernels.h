@@ -66,6 +66,7 @@ struct LocalArgs {
   int ovfCap;            /* >= ldsWords / 4                                    */
   int ldsWords;          /* dynamic LDS of the launch, in 4-byte words          */
   int shape;             /* work-group shape, [0, dlp_local_shape_count())      */
+  int noOwner;           /* 1: queue iterations only (no owner mode)            */
 };
 /* allocation sizes of the LocalArgs buffers for ldsWords words of LDS */
 inline int dlp_local_list_cap(int ldsWords) { return ldsWords / 4 + 64; }

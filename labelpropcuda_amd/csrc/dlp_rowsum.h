@@ -41,35 +41,13 @@ __device__ __forceinline__ void slot_accumulate(const Pos &pos, int len, int x, 
   }
 }
 
-/* W and S of one row, valid in the group's lane 0 (gl == 0).  Every lane of
- * the group must call it; `len` is the row length (group-uniform). */
-template <int G, class Pos>
-__device__ __forceinline__ void group_row_sums(int gl, int len, const Pos &pos, float &W,
-                                               float &S) {
+/* strides 32 .. 1 of the tree over the 64 slot registers of a group of G
+ * lanes (register j of lane gl is slot gl + G*j); the result is valid in the
+ * group's lane 0 */
+template <int G>
+__device__ __forceinline__ void group_tree(float (&ws)[64 / G], float (&ss)[64 / G], float &W,
+                                           float &S) {
   constexpr int J = 64 / G;
-  float ws[J], ss[J];
-  if (len <= 64) {
-#pragma unroll
-    for (int j = 0; j < J; ++j) {
-      const int p = gl + G * j;
-      float w = 0.0f, s = 0.0f;
-      if (p < len) pos(p, w, s);
-      ws[j] = w;
-      ss[j] = s;
-    }
-  } else {  // general form: four slots per register (strides 128, 64)
-#pragma unroll
-    for (int j = 0; j < J; ++j) {
-      const int t = gl + G * j;
-      float w0, s0, w1, s1, w2, s2, w3, s3;
-      slot_accumulate(pos, len, t, w0, s0);
-      slot_accumulate(pos, len, t + 64, w1, s1);
-      slot_accumulate(pos, len, t + 128, w2, s2);
-      slot_accumulate(pos, len, t + 192, w3, s3);
-      ws[j] = (w0 + w2) + (w1 + w3);
-      ss[j] = (s0 + s2) + (s1 + s3);
-    }
-  }
 #pragma unroll
   for (int h = J / 2; h >= 1; h >>= 1) {  // strides 32 .. G
 #pragma unroll
@@ -85,6 +63,51 @@ __device__ __forceinline__ void group_row_sums(int gl, int len, const Pos &pos, 
     W = W + __shfl_down(W, off, G);
     S = S + __shfl_down(S, off, G);
   }
+}
+
+/* W and S of a row of at most 64 entries (slots 64 .. 255 of the reference
+ * block hold zeros), valid in the group's lane 0.  Every lane of the group
+ * must call it; `len` is the row length (group-uniform). */
+template <int G, class Pos>
+__device__ __forceinline__ void group_short_row_sums(int gl, int len, const Pos &pos, float &W,
+                                                     float &S) {
+  constexpr int J = 64 / G;
+  float ws[J], ss[J];
+#pragma unroll
+  for (int j = 0; j < J; ++j) {
+    const int p = gl + G * j;
+    float w = 0.0f, s = 0.0f;
+    if (p < len) pos(p, w, s);
+    ws[j] = w;
+    ss[j] = s;
+  }
+  group_tree<G>(ws, ss, W, S);
+}
+
+/* W and S of one row of any length, valid in the group's lane 0 (gl == 0).
+ * Every lane of the group must call it; `len` is the row length
+ * (group-uniform). */
+template <int G, class Pos>
+__device__ __forceinline__ void group_row_sums(int gl, int len, const Pos &pos, float &W,
+                                               float &S) {
+  if (len <= 64) {
+    group_short_row_sums<G>(gl, len, pos, W, S);
+    return;
+  }
+  constexpr int J = 64 / G;
+  float ws[J], ss[J];
+#pragma unroll
+  for (int j = 0; j < J; ++j) {  // four slots per register (strides 128, 64)
+    const int t = gl + G * j;
+    float w0, s0, w1, s1, w2, s2, w3, s3;
+    slot_accumulate(pos, len, t, w0, s0);
+    slot_accumulate(pos, len, t + 64, w1, s1);
+    slot_accumulate(pos, len, t + 128, w2, s2);
+    slot_accumulate(pos, len, t + 192, w3, s3);
+    ws[j] = (w0 + w2) + (w1 + w3);
+    ss[j] = (s0 + s2) + (s1 + s3);
+  }
+  group_tree<G>(ws, ss, W, S);
 }
 
 constexpr int bit_reverse(int x, int bits) {

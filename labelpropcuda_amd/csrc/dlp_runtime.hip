@@ -527,9 +527,11 @@ extern "C" int dlp_context_set_tuning(dlp_context *c, const dlp_tuning *t) {
     }
     n.localShape = t->localShape;
   }
+  if (t->localOwnerMode) n.localOwnerMode = t->localOwnerMode < 0 ? -1 : 1;
   c->tuning = n;
   c->local.ldsWords = n.localLdsBytes / 4;
   c->local.shape = n.localShape - 1;
+  c->local.noOwner = n.localOwnerMode < 0 ? 1 : 0;
   return DLP_OK;
 }
 
@@ -690,7 +692,7 @@ extern "C" int dlp_context_set_profiling(dlp_context *c, int enable) {
   DLP_HIP(hipMemcpy(&ctr, c->d.ctr, sizeof(ctr), hipMemcpyDeviceToHost));
   ctr.profEdges = ctr.profChangedEdges = ctr.profAppended = 0;
   ctr.profLocalUpdates = ctr.profLocalEdges = ctr.profLocalChangedEdges = 0;
-  ctr.profLocalAppended = 0;
+  ctr.profLocalAppended = ctr.profLocalOwnerUpdates = 0;
   DLP_HIP(hipMemcpy(c->d.ctr, &ctr, sizeof(ctr), hipMemcpyHostToDevice));
   c->profiling = enable;
   c->timing.perChunk = enable == 1;
@@ -719,6 +721,7 @@ extern "C" int dlp_context_prop_profile(dlp_context *c, dlp_prop_profile *out) {
   out->localEdges = (int64_t)ctr.profLocalEdges;
   out->localChangedEdges = (int64_t)ctr.profLocalChangedEdges;
   out->localAppended = (int64_t)ctr.profLocalAppended;
+  out->localOwnerUpdates = (int64_t)ctr.profLocalOwnerUpdates;
   return DLP_OK;
 }
 

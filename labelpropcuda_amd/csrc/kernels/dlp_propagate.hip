@@ -145,8 +145,29 @@ __global__ __launch_bounds__(kPT) void k_propagate(PropArgs a, int k) {
         len = a.rowLen ? a.rowLen[u] : a.rowEnd[u] - beg;
       }
     }
+    /* rows of up to 64 entries by the lane group; longer ones (rare at the
+     * degrees of these graphs) by the whole wave, one after another, so the
+     * group path needs no registers for the 256-slot form and the kernel
+     * keeps a high occupancy */
+    const bool longRow = act && len > 64;
     float W, S;
-    dlp::group_row_sums<G>(gl, len, GlobalPos{a.colInd, a.values, Fold, a.status, beg}, W, S);
+    dlp::group_short_row_sums<G>(gl, longRow ? 0 : len,
+                                 GlobalPos{a.colInd, a.values, Fold, a.status, beg}, W, S);
+    unsigned long long longMask = dlp_ballot(longRow && gl == 0);
+    while (longMask) {  // wave-uniform
+      const int src = __ffsll((long long)longMask) - 1;
+      longMask &= longMask - 1;
+      const int lbeg = __shfl(beg, src, 64), llen = __shfl(len, src, 64);
+      float LW, LS;
+      dlp::group_row_sums<64>(lane, llen, GlobalPos{a.colInd, a.values, Fold, a.status, lbeg},
+                              LW, LS);
+      LW = __shfl(LW, 0, 64);
+      LS = __shfl(LS, 0, 64);
+      if (lane == src) {
+        W = LW;
+        S = LS;
+      }
+    }
     int changed = 0;
     if (act && gl == 0) {
       const float oldLabel = Fold[u];
@@ -293,7 +314,7 @@ __device__ int block_scan_lds(const int *a, int *out, int n, int *tmp /* >= 17 i
 
 enum {
   SH_GEN, SH_CC, SH_NCAND, SH_NCORE, SH_BIG, SH_REBUILD, SH_NOVF, SH_HASLONG,
-  SH_NHE, SH_NHI, SH_QC0, SH_QC1, SH_QC2, SH_NLONG0, SH_NLONG1, SH_COUNT
+  SH_NHE, SH_NHI, SH_QC0, SH_QC1, SH_QC2, SH_NLONG0, SH_NLONG1, SH_INELIG, SH_COUNT
 };
 
 /* state of one local iteration shared by its row paths */
@@ -422,12 +443,303 @@ struct LocalShared {
   int tmp[17];
   int dummyStamp[64]; /* INT_MAX, set by the kernel before the first pass */
   int dummyOut[64];
+  int ownerCount;     /* queue rebuilt from the owner-mode flags */
+  unsigned long long ownerSum; /* frontier sizes of the owner-mode iterations */
 };
 
 /* lane-local work counters of the local passes */
 struct LocalProf {
-  unsigned U = 0, E = 0, C = 0, A = 0;
+  unsigned U = 0, E = 0, C = 0, A = 0, D = 0; /* D: updates run in owner mode */
 };
+
+/* One owned vertex per lane in owner mode (all 64 lanes call it; `act`
+ * lanes hold a flagged vertex whose row has at most L entries, stored from
+ * `row`, padded to L records unless MASKED; `row` points at a stored row
+ * for every lane).  The label arithmetic is lane_rows'; the next frontier is
+ * set with plain flag stores (every core vertex is eligible in owner mode,
+ * and so is the vertex itself).  Branch-free like lane_rows: loads come from
+ * stored rows only (a masked position past the row reads the row's first
+ * record and is then treated as padding), and a lane with nothing to store
+ * at a step stores to its dummy LDS word. */
+template <int L, bool MASKED>
+__device__ __forceinline__ void owner_rows(const PropArgs &a, const LocalArgs &l,
+                                           const LocalIter &it, const int2 *row, int nullSlot,
+                                           const float *Fo, float *Fn, int *nxt, int *dumOut,
+                                           bool act, int ci, int len, bool hasOut,
+                                           float oldLabel, float &nl, bool &changed,
+                                           LocalProf &prof) {
+  /* all record loads, then all label loads, then the sums: two LDS round
+   * trips for the row */
+  int tt[L];
+  float ww[L], ff[L];
+#pragma unroll
+  for (int j = 0; j < L; ++j) {
+    const bool in = act && (!MASKED || j < len);
+    const int2 r = row[(!MASKED || in) ? j : 0];
+    tt[j] = in ? r.x : nullSlot;
+    ww[j] = in ? __int_as_float(r.y) : 0.0f;
+  }
+#pragma unroll
+  for (int j = 0; j < L; ++j) ff[j] = Fo[tt[j]];
+  float W, S;
+  dlp::lane_row_sums<L>(
+      L,
+      [&](int j, float &w, float &s) {
+        w = ww[j];
+        s = ww[j] * ff[j];
+      },
+      W, S);
+  nl = dlp::new_label(W, S, oldLabel);
+  changed = act && fabsf(nl - oldLabel) > a.delta;
+  *(act ? &Fn[ci] : (float *)dumOut) = nl;
+  prof.E += act ? (unsigned)len : 0u;
+  prof.C += changed ? (unsigned)len + 1u : 0u;
+#pragma unroll
+  for (int j = 0; j < L; ++j) *((changed && tt[j] < it.nCore) ? &nxt[tt[j]] : dumOut) = 1;
+  *(changed ? &nxt[ci] : dumOut) = 1;
+  if (__any(changed && hasOut)) {  // wave-uniform
+    if (changed && hasOut) {
+#pragma unroll
+      for (int j = 0; j < L; ++j) {
+        const int h = tt[j] - it.nCore;
+        if ((unsigned)h < (unsigned)it.nHE) prof.A += mark_outside(a, l, it, it.haloE[h]);
+      }
+    }
+  }
+}
+
+/* Owner mode of the LDS-resident iterations, for a working set whose core
+ * has at most V vertices per thread and holds eligible vertices only (as it
+ * does when it is made of frontier vertices).  Thread tid owns core vertices
+ * tid + v*LT and keeps in registers their row length and place and both
+ * label slots; the frontier is a flag
+ * word per core vertex (the two queue arrays serve as the flags of
+ * iterations k and k+1).  An iteration reads the owned flags, loads the rows
+ * and neighbour labels of flagged vertices, writes the new labels and, for
+ * changed vertices, sets the next flags with plain stores (a vertex flagged
+ * by several neighbours is still one vertex); rows longer than kLaneRowMax
+ * are done by the whole wave, one after another.  The frontier size is a
+ * per-wave count added before the iteration's one barrier.  No returning
+ * atomics, appends or frontier-list loads: a few LDS round trips per
+ * iteration.  Iterations and per-vertex arithmetic are the queue
+ * iterations' (same sums, same Jacobi buffers), so labels, frontiers and
+ * counts are identical.  On exit the flags of iteration k become its queue
+ * again and the stamps of the vertices that entered a frontier in the pass
+ * are set, as the queue iterations leave them.  Returns the DLP_LOCAL_*
+ * reason, or 0 when an eligible vertex outside the core was marked
+ * (rebuild). */
+template <int LT, int SR, int V>
+__device__ __forceinline__ int owner_iterations(
+    const PropArgs &a, const LocalArgs &l, LocalShared &S, int &k, int maxIter, int kBuild,
+    int nCore, int nHE, int nullSlot, int stampBase, const int *rlL, const int *rsL,
+    const int2 *recS, const int2 *recL, int *stampL, int *q0L, int *q1L, float *lab0L,
+    float *lab1L, const int *haloE, unsigned long long &passUpdates, LocalProf &prof) {
+  constexpr int RS = SR + 1;
+  int *sh = S.sh;
+  DevCounters *c = a.ctr;
+  const int tid = threadIdx.x, lane = tid & 63;
+  int *dumOut = S.dummyOut + lane;
+  /* the queue of iteration k becomes flags */
+  int qv[V];
+  const int nq = sh[SH_QC0 + k % 3];
+  {
+    const int *qin = (k & 1) ? q1L : q0L;
+#pragma unroll
+    for (int v = 0; v < V; ++v) qv[v] = tid + v * LT < nq ? qin[tid + v * LT] : -1;
+  }
+  __syncthreads();  // the queue and its count are read
+#pragma unroll
+  for (int v = 0; v < V; ++v) {
+    const int ci = tid + v * LT;
+    if (ci < nCore) {
+      q0L[ci] = 0;
+      q1L[ci] = 0;
+    }
+  }
+  if (tid == 0) {
+    sh[SH_QC0] = sh[SH_QC0 + 1] = sh[SH_QC0 + 2] = 0;
+    S.ownerSum = 0ull;
+  }
+  __syncthreads();
+  {
+    int *cur = (k & 1) ? q1L : q0L;
+#pragma unroll
+    for (int v = 0; v < V; ++v)
+      if (qv[v] >= 0) cur[qv[v]] = 1;
+  }
+  /* the owned vertices: row length and record offset (in recS for rows of
+   * at most SR entries, else in recL), label slots */
+  float l0[V], l1[V];
+  int len[V], ro[V];
+  bool own[V], hasOut[V], entered[V];
+#pragma unroll
+  for (int v = 0; v < V; ++v) {
+    const int ci = tid + v * LT;
+    own[v] = ci < nCore;
+    len[v] = ro[v] = 0;
+    hasOut[v] = entered[v] = false;
+    l0[v] = l1[v] = 0.0f;
+    if (own[v]) {
+      const int rl = rlL[ci];
+      len[v] = rl & kRowLenMask;
+      hasOut[v] = (rl & kRowHasOut) != 0;
+      ro[v] = len[v] <= SR ? ci * RS : rsL[ci];
+      l0[v] = lab0L[ci];
+      l1[v] = lab1L[ci];
+    }
+  }
+  __syncthreads();  // flags of iteration k are set
+
+  int reason = 0;
+  while (true) {
+    const int kb = k & 1;
+    int *cur = kb ? q1L : q0L;
+    int *nxt = kb ? q0L : q1L;
+    const float *Fo = kb ? lab1L : lab0L;
+    float *Fn = kb ? lab0L : lab1L;
+    bool f[V];
+    int fl[V];
+    int cnt = 0;
+#pragma unroll
+    for (int v = 0; v < V; ++v) fl[v] = cur[own[v] ? tid + v * LT : 0];  // loads in flight together
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      f[v] = own[v] && fl[v] != 0;
+      cnt += __popcll(dlp_ballot(f[v]));
+    }
+    if (lane == 0 && cnt) atomicAdd(&sh[SH_QC0 + k % 3], cnt);
+    if (k >= maxIter) {  // uniform: the size of frontier k decides the reason
+      __syncthreads();
+      reason = sh[SH_QC0 + k % 3] ? DLP_LOCAL_CAPPED : DLP_LOCAL_CONVERGED;
+      break;
+    }
+    /* slot (k+1) % 3 was last read after the barrier of iteration k-2 */
+    if (tid == 0) sh[SH_QC0 + (k + 1) % 3] = 0;
+    const LocalIter it{stampL, nullptr, nullptr, sh, haloE, nullptr, dumOut, nCore, nHE,
+                       k + 1, stampBase + k + 1, stampBase};
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      if (!__any(f[v])) continue;  // wave-uniform
+      const int ci = own[v] ? tid + v * LT : 0;
+      const float oldLabel = kb ? l1[v] : l0[v];
+      /* rows up to kLaneRowMax: one lane per vertex */
+      const bool lanePath = f[v] && len[v] <= kLaneRowMax;
+      if (__any(lanePath)) {  // wave-uniform
+        float nl = 0.0f;
+        bool changed = false;
+        /* a stored row for every lane: the lane's own short row, else row 0 */
+        const int2 *rowS = recS + (len[v] <= SR ? ro[v] : 0);
+        if (!__any(lanePath && len[v] > 4))
+          owner_rows<(SR > 4 ? 4 : SR), false>(a, l, it, rowS, nullSlot, Fo, Fn, nxt, dumOut,
+                                               lanePath, ci, len[v], hasOut[v], oldLabel, nl,
+                                               changed, prof);
+        else if (!__any(lanePath && len[v] > SR))
+          owner_rows<SR, false>(a, l, it, rowS, nullSlot, Fo, Fn, nxt, dumOut, lanePath, ci,
+                                len[v], hasOut[v], oldLabel, nl, changed, prof);
+        else
+          owner_rows<kLaneRowMax, true>(a, l, it, len[v] <= SR ? rowS : recL + ro[v], nullSlot,
+                                        Fo, Fn, nxt, dumOut, lanePath, ci, len[v], hasOut[v],
+                                        oldLabel, nl, changed, prof);
+        l0[v] = (lanePath && kb) ? nl : l0[v];
+        l1[v] = (lanePath && !kb) ? nl : l1[v];
+      }
+      /* longer rows: the wave takes them one by one */
+      unsigned long long longMask = dlp_ballot(f[v] && len[v] > kLaneRowMax);
+      while (longMask) {  // wave-uniform
+        const int src = __ffsll((long long)longMask) - 1;
+        longMask &= longMask - 1;
+        const int lci = __shfl(ci, src, 64), llen = __shfl(len[v], src, 64),
+                  lro = __shfl(ro[v], src, 64);
+        const float lold = __shfl(oldLabel, src, 64);
+        float W, Sm;
+        dlp::group_row_sums<64>(lane, llen, LocalPos{recL, Fo, lro}, W, Sm);
+        float lnl = 0.0f;
+        int changed = 0;
+        if (lane == 0) {
+          lnl = dlp::new_label(W, Sm, lold);
+          Fn[lci] = lnl;
+          changed = fabsf(lnl - lold) > a.delta;
+          prof.E += (unsigned)llen;
+          if (changed) prof.C += (unsigned)llen + 1u;
+        }
+        lnl = __shfl(lnl, 0, 64);
+        changed = __shfl(changed, 0, 64);
+        if (lane == src) {
+          if (kb)
+            l0[v] = lnl;
+          else
+            l1[v] = lnl;
+        }
+        if (!changed) continue;  // wave-uniform
+        for (int p0 = 0; p0 <= llen; p0 += 64) {  // wave-uniform
+          const int p = p0 + lane;
+          if (p < llen) {
+            const int t = recL[lro + p].x;
+            if (t < nCore) {
+              nxt[t] = 1;
+            } else if (t < nCore + nHE) {
+              prof.A += mark_outside(a, l, it, haloE[t - nCore]);
+            }
+          } else if (p == llen) {
+            nxt[lci] = 1;  // itself: flagged, hence eligible
+          }
+        }
+      }
+      *(f[v] ? &cur[ci] : dumOut) = 0;
+      entered[v] = entered[v] || (f[v] && k > kBuild);
+    }
+    __syncthreads();
+    const int n = sh[SH_QC0 + k % 3];
+    if (n == 0) {  // uniform
+      reason = DLP_LOCAL_CONVERGED;
+      break;
+    }
+    if (tid == 0) {  // sums kept in LDS: no registers live across the loop
+      if (k == 0) c->frontier0 = n;
+      if (a.iterSizes && k < a.iterSizesCap) a.iterSizes[k] = n;
+      atomicAdd(&S.ownerSum, (unsigned long long)n);
+    }
+    ++k;
+    if (sh[SH_REBUILD]) break;  // uniform after the barrier
+  }
+
+  /* flags of iteration k -> its queue; stamps of the vertices that entered */
+  {
+    int *cur = (k & 1) ? q1L : q0L;
+    bool f[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      f[v] = own[v] && cur[tid + v * LT] != 0;
+      entered[v] = entered[v] || (f[v] && k > kBuild);
+    }
+    if (tid == 0) S.ownerCount = 0;
+    __syncthreads();  // every flag is read
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      const unsigned long long m = dlp_ballot(f[v]);
+      int base = 0;
+      if (lane == 0 && m) base = atomicAdd(&S.ownerCount, __popcll(m));
+      base = __shfl(base, 0, 64);
+      if (f[v]) cur[base + dlp_rank_below(m)] = tid + v * LT;
+      if (entered[v]) stampL[tid + v * LT] = k;  // k > kBuild
+    }
+    __syncthreads();
+    if (tid == 0) {
+      const int nOut = S.ownerCount;
+      sh[SH_QC0 + k % 3] = nOut;
+      /* iterations kBuild .. k-1 ran; the first one processed the nq
+       * entries of the initial queue, every later frontier and the one
+       * left for iteration k were appended in this pass */
+      const unsigned long long sum = S.ownerSum;
+      passUpdates += sum;
+      prof.U += (unsigned)sum;
+      prof.D += (unsigned)sum;
+      if (k > kBuild) prof.A += (unsigned)(sum - (unsigned long long)nq) + (unsigned)nOut;
+    }
+    __syncthreads();
+  }
+  return reason;
+}
 
 /* Runs LDS-resident passes from iteration k (updated) until the frontier
  * empties, k reaches maxIter, or the working set outgrows the LDS budget;
@@ -472,6 +784,7 @@ __device__ __forceinline__ int local_passes(const PropArgs &a, const LocalArgs &
       sh[SH_NHI] = 0;
       sh[SH_NLONG0] = 0;
       sh[SH_NLONG1] = 0;
+      sh[SH_INELIG] = 0;
     }
     __syncthreads();
     const int gen = sh[SH_GEN], cc = sh[SH_CC], nCand = sh[SH_NCAND];
@@ -555,7 +868,9 @@ __device__ __forceinline__ int local_passes(const PropArgs &a, const LocalArgs &
       const int g = gidL[i];
       lab0L[i] = a.lab0[g];
       lab1L[i] = a.lab1[g];
-      stampL[i] = (a.iterStamp[g] == DLP_INT_MAX) ? DLP_INT_MAX : -1;
+      const bool inelig = a.iterStamp[g] == DLP_INT_MAX;
+      stampL[i] = inelig ? DLP_INT_MAX : -1;
+      if (inelig) sh[SH_INELIG] = 1;
       const int beg = a.rowBeg[g], len = rlL[i];
       const bool shortRow = len <= SR;
       int2 *out = shortRow ? recS + i * RS : recL + rsL[i];
@@ -609,103 +924,114 @@ __device__ __forceinline__ int local_passes(const PropArgs &a, const LocalArgs &
     const int kBuild = k;
 
     /* ---- iterations ------------------------------------------------------- */
-    while (true) {
-      const int n = sh[SH_QC0 + k % 3];
-      if (n == 0) {
-        reason = DLP_LOCAL_CONVERGED;
-        break;
-      }
-      if (k >= maxIter) {
-        reason = DLP_LOCAL_CAPPED;
-        break;
-      }
-      const int kb = k & 1;
-      if (tid == 0) {
-        sh[SH_QC0 + (k + 2) % 3] = 0;    // appended to by iteration k+1
-        sh[SH_NLONG0 + (kb ^ 1)] = 0;    // long-row list of iteration k+1
-        if (k == 0) c->frontier0 = n;
-        if (a.iterSizes && k < a.iterSizesCap) a.iterSizes[k] = n;
-        passUpdates += (unsigned long long)n;
-        profU += (unsigned)n;
-      }
-      const int *qin = kb ? q1L : q0L;
-      const float *Fo = kb ? lab1L : lab0L;
-      float *Fn = kb ? lab0L : lab1L;
-      int *nLongCtr = &sh[SH_NLONG0 + kb];
-      const LocalIter it{stampL, kb ? q0L : q1L, &sh[SH_QC0 + (k + 1) % 3], sh, haloE,
-                         dummyStamp, dummyOut, nCore, nHE, k + 1, stampBase + k + 1,
-                         stampBase};
+    constexpr int OV = 2048 / LT;  // owned vertices per thread in owner mode: cores up to 2048
+    if (!l.noOwner && nCore <= LT && !sh[SH_INELIG]) {  // uniform
+      reason = owner_iterations<LT, SR, 1>(a, l, S, k, maxIter, kBuild, nCore, nHE, nullSlot,
+                                           stampBase, rlL, rsL, recS, recL, stampL, q0L, q1L,
+                                           lab0L, lab1L, haloE, passUpdates, prof);
+    } else if (!l.noOwner && nCore <= OV * LT && !sh[SH_INELIG]) {  // uniform
+      reason = owner_iterations<LT, SR, OV>(a, l, S, k, maxIter, kBuild, nCore, nHE, nullSlot,
+                                            stampBase, rlL, rsL, recS, recL, stampL, q0L, q1L,
+                                            lab0L, lab1L, haloE, passUpdates, prof);
+    } else {  // queue iterations
+      while (true) {
+        const int n = sh[SH_QC0 + k % 3];
+        if (n == 0) {
+          reason = DLP_LOCAL_CONVERGED;
+          break;
+        }
+        if (k >= maxIter) {
+          reason = DLP_LOCAL_CAPPED;
+          break;
+        }
+        const int kb = k & 1;
+        if (tid == 0) {
+          sh[SH_QC0 + (k + 2) % 3] = 0;    // appended to by iteration k+1
+          sh[SH_NLONG0 + (kb ^ 1)] = 0;    // long-row list of iteration k+1
+          if (k == 0) c->frontier0 = n;
+          if (a.iterSizes && k < a.iterSizesCap) a.iterSizes[k] = n;
+          passUpdates += (unsigned long long)n;
+          profU += (unsigned)n;
+        }
+        const int *qin = kb ? q1L : q0L;
+        const float *Fo = kb ? lab1L : lab0L;
+        float *Fn = kb ? lab0L : lab1L;
+        int *nLongCtr = &sh[SH_NLONG0 + kb];
+        const LocalIter it{stampL, kb ? q0L : q1L, &sh[SH_QC0 + (k + 1) % 3], sh, haloE,
+                           dummyStamp, dummyOut, nCore, nHE, k + 1, stampBase + k + 1,
+                           stampBase};
 
-      /* rows up to kLaneRowMax: one lane per frontier vertex */
-      for (int base = 0; base < n; base += LT) {  // block-uniform
-        if (base + (tid & ~63) >= n) continue;     // wave-uniform: no vertex for this wave
-        const int item = base + tid;
-        const bool act = item < n;
-        const int ci = act ? qin[item] : 0;
-        const int rl = act ? rlL[ci] : 0;
-        const int len = rl & kRowLenMask;
-        const bool hasOut = (rl & kRowHasOut) != 0;
-        const bool shortRow = act && len <= SR;
-        const bool midRow = act && len > SR && len <= kLaneRowMax;
-        if (hasLong) {
-          const int slot = dlp_wave_append(nLongCtr, act && len > kLaneRowMax);
-          if (slot >= 0) longL[slot] = ci;
-        }
-        if (__any(shortRow)) {  // wave-uniform
-          const int2 *row = recS + ci * RS;
-          if (SR > 4 && !__any(shortRow && len > 4))
-            lane_rows<(SR > 4 ? 4 : SR), false>(a, l, it, row, nullSlot, Fo, Fn, shortRow, ci,
-                                                len, hasOut, profE, profC, profA);
-          else
-            lane_rows<SR, false>(a, l, it, row, nullSlot, Fo, Fn, shortRow, ci, len, hasOut,
-                                 profE, profC, profA);
-        }
-        if (SR < kLaneRowMax && __any(midRow)) {  // wave-uniform
-          const int2 *row = recL + (midRow ? rsL[ci] : 0);
-          lane_rows<kLaneRowMax, true>(a, l, it, row, nullSlot, Fo, Fn, midRow, ci,
-                                       midRow ? len : 0, hasOut, profE, profC, profA);
-        }
-      }
-      /* longer rows: one wave per frontier vertex */
-      if (hasLong) {  // uniform
-        __syncthreads();
-        const int nLong = *nLongCtr;
-        for (int i = wave; i < nLong; i += LT / 64) {  // wave-uniform
-          const int ci = longL[i];
-          const int rs = rsL[ci], len = rlL[ci] & kRowLenMask;
-          float W, S;
-          dlp::group_row_sums<64>(lane, len, LocalPos{recL, Fo, rs}, W, S);
-          int changed = 0;
-          if (lane == 0) {
-            const float oldLabel = Fo[ci];
-            const float nl = dlp::new_label(W, S, oldLabel);
-            Fn[ci] = nl;
-            changed = fabsf(nl - oldLabel) > a.delta;
-            profE += (unsigned)len;
-            if (changed) profC += (unsigned)len + 1u;
+        /* rows up to kLaneRowMax: one lane per frontier vertex */
+        for (int base = 0; base < n; base += LT) {  // block-uniform
+          if (base + (tid & ~63) >= n) continue;     // wave-uniform: no vertex for this wave
+          const int item = base + tid;
+          const bool act = item < n;
+          const int ci = act ? qin[item] : 0;
+          const int rl = act ? rlL[ci] : 0;
+          const int len = rl & kRowLenMask;
+          const bool hasOut = (rl & kRowHasOut) != 0;
+          const bool shortRow = act && len <= SR;
+          const bool midRow = act && len > SR && len <= kLaneRowMax;
+          if (hasLong) {
+            const int slot = dlp_wave_append(nLongCtr, act && len > kLaneRowMax);
+            if (slot >= 0) longL[slot] = ci;
           }
-          changed = __shfl(changed, 0, 64);
-          if (!changed) continue;  // wave-uniform
-          for (int p0 = 0; p0 <= len; p0 += 64) {  // wave-uniform
-            const int p = p0 + lane;
-            int t = nullSlot;
-            if (p < len) t = recL[rs + p].x;
-            else if (p == len) t = ci;  // the vertex itself
-            bool appL = false;
-            if (t < nCore) {
-              appL = atomicMax(&stampL[t], it.stampNextL) < it.stampNextL;
-            } else if (t < nCore + nHE) {
-              profA += mark_outside(a, l, it, haloE[t - nCore]);
+          if (__any(shortRow)) {  // wave-uniform
+            const int2 *row = recS + ci * RS;
+            if (SR > 4 && !__any(shortRow && len > 4))
+              lane_rows<(SR > 4 ? 4 : SR), false>(a, l, it, row, nullSlot, Fo, Fn, shortRow, ci,
+                                                  len, hasOut, profE, profC, profA);
+            else
+              lane_rows<SR, false>(a, l, it, row, nullSlot, Fo, Fn, shortRow, ci, len, hasOut,
+                                   profE, profC, profA);
+          }
+          if (SR < kLaneRowMax && __any(midRow)) {  // wave-uniform
+            const int2 *row = recL + (midRow ? rsL[ci] : 0);
+            lane_rows<kLaneRowMax, true>(a, l, it, row, nullSlot, Fo, Fn, midRow, ci,
+                                         midRow ? len : 0, hasOut, profE, profC, profA);
+          }
+        }
+        /* longer rows: one wave per frontier vertex */
+        if (hasLong) {  // uniform
+          __syncthreads();
+          const int nLong = *nLongCtr;
+          for (int i = wave; i < nLong; i += LT / 64) {  // wave-uniform
+            const int ci = longL[i];
+            const int rs = rsL[ci], len = rlL[ci] & kRowLenMask;
+            float W, S;
+            dlp::group_row_sums<64>(lane, len, LocalPos{recL, Fo, rs}, W, S);
+            int changed = 0;
+            if (lane == 0) {
+              const float oldLabel = Fo[ci];
+              const float nl = dlp::new_label(W, S, oldLabel);
+              Fn[ci] = nl;
+              changed = fabsf(nl - oldLabel) > a.delta;
+              profE += (unsigned)len;
+              if (changed) profC += (unsigned)len + 1u;
             }
-            profA += appL ? 1u : 0u;
-            const int slot = dlp_wave_append(it.qcOut, appL);
-            if (slot >= 0) it.qout[slot] = t;
+            changed = __shfl(changed, 0, 64);
+            if (!changed) continue;  // wave-uniform
+            for (int p0 = 0; p0 <= len; p0 += 64) {  // wave-uniform
+              const int p = p0 + lane;
+              int t = nullSlot;
+              if (p < len) t = recL[rs + p].x;
+              else if (p == len) t = ci;  // the vertex itself
+              bool appL = false;
+              if (t < nCore) {
+                appL = atomicMax(&stampL[t], it.stampNextL) < it.stampNextL;
+              } else if (t < nCore + nHE) {
+                profA += mark_outside(a, l, it, haloE[t - nCore]);
+              }
+              profA += appL ? 1u : 0u;
+              const int slot = dlp_wave_append(it.qcOut, appL);
+              if (slot >= 0) it.qout[slot] = t;
+            }
           }
         }
+        __syncthreads();
+        ++k;
+        if (sh[SH_REBUILD]) break;  // uniform after the barrier
       }
-      __syncthreads();
-      ++k;
-      if (sh[SH_REBUILD]) break;  // uniform after the barrier
     }
 
     /* ---- write back: labels, touched, next frontier, core list ---------- */
@@ -779,6 +1105,8 @@ __device__ __forceinline__ void local_prof_flush(const PropArgs &a, LocalProf &p
     if (prof.A) atomicAdd(&c->profLocalAppended, (unsigned long long)prof.A);
   }
   if (threadIdx.x == 0 && prof.U) atomicAdd(&c->profLocalUpdates, (unsigned long long)prof.U);
+  if (threadIdx.x == 0 && prof.D)
+    atomicAdd(&c->profLocalOwnerUpdates, (unsigned long long)prof.D);
   prof = LocalProf{};
 }
 

@@ -161,12 +161,78 @@ def test_engine_mix_gives_identical_results(gpu_available, c1_inputs, threshold,
 
 
 @pytest.mark.parametrize("shape", [1, 2, 5])
-def test_engine_mix_long_rows(gpu_available, shape):
+@pytest.mark.parametrize("owner", [1, -1])
+def test_engine_mix_long_rows(gpu_available, shape, owner):
     """Average degree 40: rows longer than 32 take the wave-per-row paths of
-    both kernels (and the general row sum beyond 64 entries)."""
+    both kernels (and the general row sum beyond 64 entries), in owner mode
+    and in queue iterations."""
     inp = L.generate(3000, 40, 6, 7)
-    prof = run_engine_mix(inp, 2000, localThreshold=1 << 30, localShape=shape)
+    prof = run_engine_mix(inp, 2000, localThreshold=1 << 30, localShape=shape,
+                          localOwnerMode=owner)
     assert prof["localLaunches"] > 0
+    assert (prof["localOwnerUpdates"] > 0) == (owner > 0)
+
+
+@pytest.mark.parametrize("shape", [1, 2, 3, 4, 5])
+@pytest.mark.parametrize("owner,threshold,ldsBytes", [(1, 1 << 30, 0), (-1, 1 << 30, 0),
+                                                      (1, 64, 0), (1, 1 << 30, 8192)])
+def test_owner_mode_and_queue_iterations(gpu_available, c1_inputs, shape, owner, threshold,
+                                         ldsBytes):
+    """The LDS-resident kernel's owner mode (one thread per core vertex,
+    frontier as flags) and its queue iterations leave the oracle's labels,
+    iteration counts and frontier sizes for every work-group shape, with
+    frequent engine switches and with core rebuilds in a small LDS budget."""
+    prof = run_engine_mix(c1_inputs, 3000, localThreshold=threshold, localLdsBytes=ldsBytes,
+                          localShape=shape, localOwnerMode=owner)
+    assert prof["localLaunches"] > 0
+    assert (prof["localOwnerUpdates"] > 0) == (owner > 0)
+    assert prof["localOwnerUpdates"] <= prof["localUpdates"]
+
+
+def hub_inputs(tmp_path):
+    """3,000 vertices at degree ~6 plus two hubs (vertex 0 adjacent to 1..700,
+    vertex 1 to 2..320), read back through the graph text files, with the
+    reference batch generator over it: once most vertices are active the
+    hubs' rows exceed 256 entries."""
+    rng = np.random.default_rng(3)
+    n = 3000
+    pairs = set()
+    for u, v in rng.integers(0, n, (9000, 2)).tolist():
+        if u != v:
+            pairs.add((min(u, v), max(u, v)))
+    pairs |= {(0, v) for v in range(1, 701)} | {(1, v) for v in range(2, 321)}
+    rows = [[] for _ in range(n)]
+    for u, v in sorted(pairs):
+        w = float(rng.uniform(0.01, 1.0))
+        rows[u].append((v, w))
+        rows[v].append((u, w))
+    prefix = str(tmp_path / "hub_")
+    rowPtr = np.cumsum([0] + [len(r) for r in rows])
+    with open(prefix + "RowPtr.txt", "w") as f:
+        f.write("\n".join(str(x) for x in rowPtr) + "\n")
+    ordered = [sorted(r) for r in rows]
+    with open(prefix + "ColInd.txt", "w") as f:
+        f.write("\n".join(str(v) for r in ordered for v, _ in r) + "\n")
+    with open(prefix + "Values.txt", "w") as f:
+        f.write("\n".join(f"{w:.6f}" for r in ordered for _, w in r) + "\n")
+    labels = rng.permutation(np.arange(n) % 2)
+    with open(prefix + "TrueLabels.txt", "w") as f:
+        f.write("\n".join(f"{i} {c}" for i, c in enumerate(labels)) + "\n")
+    return L.dynlp.from_graph_files(prefix, 5, 42)
+
+
+@pytest.mark.parametrize("lanes,threshold", [(4, -1), (8, -1), (16, -1), (64, -1), (8, 0)])
+def test_rows_beyond_64_and_256_entries(gpu_available, tmp_path, lanes, threshold):
+    """The multi-block kernel sums rows of more than 64 entries with the whole
+    wave (and rows past 256 entries with two positions per slot) while the
+    lane groups sum the short rows: oracle labels, iterations and frontiers
+    for every group width, with and without the LDS-resident kernel."""
+    inp = hub_inputs(tmp_path)
+    deg = np.diff(np.asarray(inp.rowPtr))
+    assert deg[0] >= 700 and deg[1] >= 320 and np.median(deg) < 16
+    run_engine_mix(inp, 3000, groupLanes=lanes, localThreshold=threshold)
+    dense = L.generate(2000, 80, 4, 5)  # most rows between 65 and 121 entries
+    run_engine_mix(dense, 1000, groupLanes=lanes, localThreshold=threshold)
 
 
 def test_replay_is_deterministic(gpu_available, c1_inputs):
