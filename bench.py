@@ -412,7 +412,7 @@ def stream_main(args) -> int:
     cfg = L.make_config(inputs)
     ctx = L.Context(inputs, cfg)
     ctx.set_tuning(args.lanes, args.blocks, args.chunk, args.local_threshold,
-                   localShape=args.local_shape, localOwnerMode=args.local_owner)
+                   localShape=args.local_shape, localOwnerAfter=args.local_owner_after)
     progress(f"context resident, {ctx.device_bytes() / 2**30:.2f} GiB, tuning {ctx.tuning()}")
     ctx.set_profiling(1)
     ctx.reset()
@@ -580,8 +580,9 @@ def main() -> int:
                     help="frontier size for the LDS-resident kernel (<0 disables)")
     ap.add_argument("--local-shape", type=int, default=0,
                     help="work-group shape of the LDS-resident kernel (1-based)")
-    ap.add_argument("--local-owner", type=int, default=0,
-                    help="owner mode of the LDS-resident kernel: 1 on (default), -1 off")
+    ap.add_argument("--local-owner-after", type=int, default=0,
+                    help="iterations of an LDS-resident pass before owner mode (0 = default, "
+                         "-1 = never)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--workload", choices=("c2", "batched", "serving", "irlp", "stream"),
                     default="c2",
@@ -641,7 +642,7 @@ def main() -> int:
     cfg = L.make_config(inputs)
     ctx = L.Context(inputs, cfg)
     ctx.set_tuning(args.lanes, args.blocks, args.chunk, args.local_threshold,
-                   localShape=args.local_shape, localOwnerMode=args.local_owner)
+                   localShape=args.local_shape, localOwnerAfter=args.local_owner_after)
     tuning = ctx.tuning()
 
     for _ in range(args.warmup):

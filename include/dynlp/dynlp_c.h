@@ -229,10 +229,12 @@ typedef struct dlp_tuning {
   int localShape;       /* work-group shape of the local kernel, 1-based
                            (threads / records per padded row): 1 = 1024/8,
                            2 = 1024/16, 3 = 512/8, 4 = 256/8, 5 = 1024/4 */
-  int localOwnerMode;   /* owner mode of the local kernel (one thread per
-                           core vertex, row in registers, frontier as
-                           flags) when the core fits: 1 on (default),
-                           -1 off (queue iterations only)                */
+  int localOwnerAfter;  /* iterations one launch of the local kernel runs
+                           on its frontier queue before the propagation
+                           goes on with the owner-mode local kernel (one
+                           thread per core vertex, frontier as flags;
+                           for long-lived frontiers such as limit
+                           cycles): 1024 by default, -1 never            */
 } dlp_tuning;
 
 /* Uploads the full graph, true labels and every batch, allocates all device
@@ -308,6 +310,7 @@ typedef struct dlp_prop_profile {
   int64_t localChangedEdges;
   int64_t localAppended;
   int64_t localOwnerUpdates; /* of localUpdates: processed in owner mode   */
+  int64_t localBuilds;       /* LDS working sets built (passes)           */
 } dlp_prop_profile;
 
 /* Sets the profiling level and clears the accumulated profile:

@@ -100,7 +100,7 @@ class Tuning(C.Structure):
         ("localThreshold", C.c_int),
         ("localLdsBytes", C.c_int),
         ("localShape", C.c_int),
-        ("localOwnerMode", C.c_int),
+        ("localOwnerAfter", C.c_int),
     ]
 
 
@@ -156,6 +156,7 @@ class PropProfile(C.Structure):
         ("localChangedEdges", C.c_int64),
         ("localAppended", C.c_int64),
         ("localOwnerUpdates", C.c_int64),
+        ("localBuilds", C.c_int64),
     ]
 
 

@@ -36,7 +36,7 @@ struct dlp_context {
    * updates/s: fewer host round trips, and after a working set outgrew LDS
    * the LDS kernel is retried only after a chunk); local threshold and LDS
    * set at creation */
-  dlp_tuning tuning{8, 0, 1024, 0, 0, 1, 1};
+  dlp_tuning tuning{8, 0, 1024, 0, 0, 1, kOwnerAfterDefault};
   LocalArgs local{};
   int localMaxBytes = 0;
   int nextBatch = 0;

@@ -67,13 +67,16 @@ struct DevCounters {
   unsigned long long profLocalChangedEdges;
   unsigned long long profLocalAppended;
   unsigned long long profLocalOwnerUpdates; /* of profLocalUpdates: owner mode */
+  unsigned long long profLocalBuilds;       /* LDS working sets built         */
 };
 
 /* exit reasons of the LDS-resident propagation kernel */
 enum {
   DLP_LOCAL_CONVERGED = 1, /* next frontier empty                            */
   DLP_LOCAL_CAPPED = 2,    /* maxIterations reached                          */
-  DLP_LOCAL_TOO_BIG = 3    /* working set does not fit; continue multi-block */
+  DLP_LOCAL_TOO_BIG = 3,   /* working set does not fit; continue multi-block */
+  DLP_LOCAL_LONG = 4       /* queue kernel ran LocalArgs::ownerAfter iterations:
+                              continue with the owner kernel                 */
 };
 
 /* per-batch record kept on the device, read back once per run */

@@ -66,8 +66,12 @@ struct LocalArgs {
   int ovfCap;            /* >= ldsWords / 4                                    */
   int ldsWords;          /* dynamic LDS of the launch, in 4-byte words          */
   int shape;             /* work-group shape, [0, dlp_local_shape_count())      */
-  int noOwner;           /* 1: queue iterations only (no owner mode)            */
+  int ownerAfter;        /* iterations of a launch of the queue kernel before the
+                            host goes on with the owner kernel; < 0: never     */
 };
+/* default LocalArgs::ownerAfter: only long-lived frontiers (C2's limit
+ * cycles run 100,000 iterations in one launch) go on in owner mode */
+constexpr int kOwnerAfterDefault = 1024;
 /* allocation sizes of the LocalArgs buffers for ldsWords words of LDS */
 inline int dlp_local_list_cap(int ldsWords) { return ldsWords / 4 + 64; }
 /* default frontier threshold of the local kernel: the frontier whose
@@ -85,6 +89,7 @@ inline int dlp_local_default_threshold(int ldsWords, long long meanDeg) {
  * its count, stamps, touched list, iteration counters) and records the stop
  * iteration and reason in ctr->nextIter / ctr->localReason. */
 int dlp_launch_propagate_local(const PropArgs &a, const LocalArgs &l, int k, int maxIter,
+                               bool owner,
                                hipStream_t s);
 /* dynamic LDS bytes the local kernel can use on the current device */
 int dlp_local_lds_bytes(int *bytes);

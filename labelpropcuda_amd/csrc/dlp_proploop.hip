@@ -80,6 +80,7 @@ int dlp_run_propagation(const PropArgs &a, const LocalArgs *l, int maxIter,
   int k = startK;
   int chunk = 2;
   int localAfter = 0; /* no local launch before this iteration (after an overflow) */
+  bool owner = false;  /* the last queue launch of the LDS kernel ran long */
   const int maxChunk = std::max(1, cfg.maxChunk);
   if ((rc = read_ctr(a.ctr, pin, s)) != DLP_OK) return rc;
   while (k < maxIter) {
@@ -92,7 +93,7 @@ int dlp_run_propagation(const PropArgs &a, const LocalArgs *l, int maxIter,
         DLP_HIP(hipMemsetAsync(&a.ctr->localGen, 0, sizeof(int), s));
       }
       if ((rc = begin_timed(timing, s, true, 1)) != DLP_OK) return rc;
-      if ((rc = dlp_launch_propagate_local(a, *l, k, maxIter, s)) != DLP_OK) return rc;
+      if ((rc = dlp_launch_propagate_local(a, *l, k, maxIter, owner, s)) != DLP_OK) return rc;
       if ((rc = end_timed(timing, s)) != DLP_OK) return rc;
       if ((rc = read_ctr(a.ctr, pin, s)) != DLP_OK) return rc;
       const int next = pin->nextIter;
@@ -103,7 +104,9 @@ int dlp_run_propagation(const PropArgs &a, const LocalArgs *l, int maxIter,
       if (pin->localReason == DLP_LOCAL_TOO_BIG) {
         localAfter = next + std::max(2, maxChunk);
         chunk = 2;
+        owner = false;
       }
+      if (pin->localReason == DLP_LOCAL_LONG) owner = true;
       k = next;
       if (pin->localReason == DLP_LOCAL_CONVERGED || pin->localReason == DLP_LOCAL_CAPPED)
         break;

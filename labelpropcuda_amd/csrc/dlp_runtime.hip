@@ -432,6 +432,7 @@ extern "C" int dlp_context_create(dlp_context **out, const dlp_config *cfg,
     LocalArgs &l = c->local;
     l.ldsWords = ldsBytes / 4;
     l.numNodes = N;
+    l.ownerAfter = c->tuning.localOwnerAfter;
     l.coreListCap = l.ovfCap = dlp_local_list_cap(l.ldsWords);
     TRY(c->alloc(&l.coreGen, N));
     TRY(c->alloc(&l.haloGen, N));
@@ -527,11 +528,11 @@ extern "C" int dlp_context_set_tuning(dlp_context *c, const dlp_tuning *t) {
     }
     n.localShape = t->localShape;
   }
-  if (t->localOwnerMode) n.localOwnerMode = t->localOwnerMode < 0 ? -1 : 1;
+  if (t->localOwnerAfter) n.localOwnerAfter = t->localOwnerAfter < 0 ? -1 : t->localOwnerAfter;
   c->tuning = n;
   c->local.ldsWords = n.localLdsBytes / 4;
   c->local.shape = n.localShape - 1;
-  c->local.noOwner = n.localOwnerMode < 0 ? 1 : 0;
+  c->local.ownerAfter = n.localOwnerAfter;
   return DLP_OK;
 }
 
@@ -692,7 +693,7 @@ extern "C" int dlp_context_set_profiling(dlp_context *c, int enable) {
   DLP_HIP(hipMemcpy(&ctr, c->d.ctr, sizeof(ctr), hipMemcpyDeviceToHost));
   ctr.profEdges = ctr.profChangedEdges = ctr.profAppended = 0;
   ctr.profLocalUpdates = ctr.profLocalEdges = ctr.profLocalChangedEdges = 0;
-  ctr.profLocalAppended = ctr.profLocalOwnerUpdates = 0;
+  ctr.profLocalAppended = ctr.profLocalOwnerUpdates = ctr.profLocalBuilds = 0;
   DLP_HIP(hipMemcpy(c->d.ctr, &ctr, sizeof(ctr), hipMemcpyHostToDevice));
   c->profiling = enable;
   c->timing.perChunk = enable == 1;
@@ -722,6 +723,7 @@ extern "C" int dlp_context_prop_profile(dlp_context *c, dlp_prop_profile *out) {
   out->localChangedEdges = (int64_t)ctr.profLocalChangedEdges;
   out->localAppended = (int64_t)ctr.profLocalAppended;
   out->localOwnerUpdates = (int64_t)ctr.profLocalOwnerUpdates;
+  out->localBuilds = (int64_t)ctr.profLocalBuilds;
   return DLP_OK;
 }
 

@@ -330,6 +330,7 @@ extern "C" int dlp_iterative_propagation(const int *rowPtr, const int *colInd,
   l.ldsWords = w->ldsBytes / 4;
   l.numNodes = numNodes;
   l.shape = 0;
+  l.ownerAfter = kOwnerAfterDefault;
   l.coreListCap = l.ovfCap = dlp_local_list_cap(l.ldsWords);
   WS_BUF(int, coreGen, 19, 2LL * numNodes);  // core generations, then halo generations
   WS_BUF(int, coreIdx, 20, numNodes);

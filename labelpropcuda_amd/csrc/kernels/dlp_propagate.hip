@@ -727,14 +727,15 @@ __device__ __forceinline__ int owner_iterations(
     if (tid == 0) {
       const int nOut = S.ownerCount;
       sh[SH_QC0 + k % 3] = nOut;
-      /* iterations kBuild .. k-1 ran; the first one processed the nq
-       * entries of the initial queue, every later frontier and the one
-       * left for iteration k were appended in this pass */
+      /* when owner iterations ran, the first processed the nq entries of the
+       * queue it started from (counted where they were appended, or not
+       * appended in this pass); every later frontier and the one left for
+       * iteration k were appended here */
       const unsigned long long sum = S.ownerSum;
       passUpdates += sum;
       prof.U += (unsigned)sum;
       prof.D += (unsigned)sum;
-      if (k > kBuild) prof.A += (unsigned)(sum - (unsigned long long)nq) + (unsigned)nOut;
+      if (sum > 0ull) prof.A += (unsigned)(sum - (unsigned long long)nq) + (unsigned)nOut;
     }
     __syncthreads();
   }
@@ -745,9 +746,9 @@ __device__ __forceinline__ int owner_iterations(
  * empties, k reaches maxIter, or the working set outgrows the LDS budget;
  * returns the DLP_LOCAL_* reason.  All LT threads of the work-group call
  * it; `pool` is the dynamic LDS (l.ldsWords words). */
-template <int LT, int SR>
+template <int LT, int SR, bool OWNER>
 __device__ __forceinline__ int local_passes(const PropArgs &a, const LocalArgs &l, int &k, int maxIter,
-                            int *pool, LocalShared &S, LocalProf &prof) {
+                            int *pool, LocalShared &S, LocalProf &prof, int longAfter) {
   static_assert(SR <= kLaneRowMax, "padded stride above the lane row limit");
   /* padded rows are SR + 1 records apart: with 2*(SR+1) words (SR a power
    * of two) the rows of neighbouring core vertices start in different LDS
@@ -763,6 +764,7 @@ __device__ __forceinline__ int local_passes(const PropArgs &a, const LocalArgs &
   bool useCore = true;
   unsigned &profU = prof.U, &profE = prof.E, &profC = prof.C, &profA = prof.A;
   unsigned long long passUpdates = 0; /* tid 0: frontier sizes of this pass */
+  const int kLaunch = k;
 
   while (true) {  // one pass = build the LDS working set, iterate, write back
     /* every thread has read the previous pass's sh[] values */
@@ -771,6 +773,7 @@ __device__ __forceinline__ int local_passes(const PropArgs &a, const LocalArgs &
     if (tid == 0) {
       const int gen = c->localGen + 1;
       c->localGen = gen;
+      if (a.profile) c->profLocalBuilds += 1ull;
       const int cc = useCore ? min(c->coreCount, l.coreListCap) : 0;
       sh[SH_GEN] = gen;
       sh[SH_CC] = cc;
@@ -924,15 +927,23 @@ __device__ __forceinline__ int local_passes(const PropArgs &a, const LocalArgs &
     const int kBuild = k;
 
     /* ---- iterations ------------------------------------------------------- */
+    /* Owner mode (the OWNER kernels) when the core fits it, else queue
+     * iterations; the queue kernels stop with DLP_LOCAL_LONG after
+     * longAfter iterations of a launch (>= 0), so that the host goes on
+     * with the owner kernel on a long-lived frontier such as a limit cycle.
+     * The two modes are separate kernels: owner mode's registers would
+     * otherwise cost the queue iterations spills. */
     constexpr int OV = 2048 / LT;  // owned vertices per thread in owner mode: cores up to 2048
-    if (!l.noOwner && nCore <= LT && !sh[SH_INELIG]) {  // uniform
-      reason = owner_iterations<LT, SR, 1>(a, l, S, k, maxIter, kBuild, nCore, nHE, nullSlot,
-                                           stampBase, rlL, rsL, recS, recL, stampL, q0L, q1L,
-                                           lab0L, lab1L, haloE, passUpdates, prof);
-    } else if (!l.noOwner && nCore <= OV * LT && !sh[SH_INELIG]) {  // uniform
-      reason = owner_iterations<LT, SR, OV>(a, l, S, k, maxIter, kBuild, nCore, nHE, nullSlot,
-                                            stampBase, rlL, rsL, recS, recL, stampL, q0L, q1L,
-                                            lab0L, lab1L, haloE, passUpdates, prof);
+    const bool ownerOK = OWNER && nCore <= OV * LT && !sh[SH_INELIG];  // uniform
+    if (ownerOK) {
+      if (nCore <= LT)
+        reason = owner_iterations<LT, SR, 1>(a, l, S, k, maxIter, kBuild, nCore, nHE, nullSlot,
+                                             stampBase, rlL, rsL, recS, recL, stampL, q0L, q1L,
+                                             lab0L, lab1L, haloE, passUpdates, prof);
+      else
+        reason = owner_iterations<LT, SR, OV>(a, l, S, k, maxIter, kBuild, nCore, nHE,
+                                              nullSlot, stampBase, rlL, rsL, recS, recL, stampL,
+                                              q0L, q1L, lab0L, lab1L, haloE, passUpdates, prof);
     } else {  // queue iterations
       while (true) {
         const int n = sh[SH_QC0 + k % 3];
@@ -942,6 +953,10 @@ __device__ __forceinline__ int local_passes(const PropArgs &a, const LocalArgs &
         }
         if (k >= maxIter) {
           reason = DLP_LOCAL_CAPPED;
+          break;
+        }
+        if (!OWNER && longAfter >= 0 && k - kLaunch >= longAfter) {  // uniform
+          reason = DLP_LOCAL_LONG;
           break;
         }
         const int kb = k & 1;
@@ -1110,7 +1125,7 @@ __device__ __forceinline__ void local_prof_flush(const PropArgs &a, LocalProf &p
   prof = LocalProf{};
 }
 
-template <int LT, int SR>
+template <int LT, int SR, bool OWNER>
 __global__ __launch_bounds__(LT) void k_propagate_local(PropArgs a, LocalArgs l, int k0,
                                                         int maxIter) {
   extern __shared__ int4 pool4[];
@@ -1118,7 +1133,8 @@ __global__ __launch_bounds__(LT) void k_propagate_local(PropArgs a, LocalArgs l,
   if (threadIdx.x < 64) S.dummyStamp[threadIdx.x] = DLP_INT_MAX;  // first pass starts with a barrier
   LocalProf prof;
   int k = k0;
-  const int reason = local_passes<LT, SR>(a, l, k, maxIter, (int *)pool4, S, prof);
+  const int reason = local_passes<LT, SR, OWNER>(a, l, k, maxIter, (int *)pool4, S, prof,
+                                                 OWNER ? -1 : l.ownerAfter);
   if (threadIdx.x == 0) {
     a.ctr->nextIter = k;
     a.ctr->localReason = reason;
@@ -1374,7 +1390,7 @@ __global__ __launch_bounds__(LT) void k_propagate_multi(const GraphProp *graphs)
         handBack = k;
         break;
       }
-      const int reason = local_passes<LT, SR>(a, l, k, maxIter, (int *)pool4, S, lprof);
+      const int reason = local_passes<LT, SR, false>(a, l, k, maxIter, (int *)pool4, S, lprof, -1);
       if (reason != DLP_LOCAL_TOO_BIG) break;
       resumeAt = k + backoff;
       wg_global_sync();
@@ -1422,7 +1438,9 @@ int dlp_local_lds_bytes(int *bytes) {
   }
   v &= ~255;
 #define DLP_SET_LDS(I, T, R)                                                       \
-  DLP_HIP(hipFuncSetAttribute((const void *)k_propagate_local<T, R>,              \
+  DLP_HIP(hipFuncSetAttribute((const void *)k_propagate_local<T, R, false>,       \
+                              hipFuncAttributeMaxDynamicSharedMemorySize, v));     \
+  DLP_HIP(hipFuncSetAttribute((const void *)k_propagate_local<T, R, true>,        \
                               hipFuncAttributeMaxDynamicSharedMemorySize, v));     \
   DLP_HIP(hipFuncSetAttribute((const void *)k_propagate_multi<T, R>,              \
                               hipFuncAttributeMaxDynamicSharedMemorySize, v));
@@ -1433,13 +1451,18 @@ int dlp_local_lds_bytes(int *bytes) {
 }
 
 int dlp_launch_propagate_local(const PropArgs &a, const LocalArgs &l, int k, int maxIter,
-                               hipStream_t s) {
+                               bool owner, hipStream_t s) {
   const size_t lds = (size_t)l.ldsWords * 4;
   switch (l.shape) {
 #define DLP_LAUNCH_LOCAL(I, T, R)                                                       \
   case I:                                                                              \
     static_assert(kLocalShapes[I].threads == T && kLocalShapes[I].stride == R, "");  \
-    hipLaunchKernelGGL((k_propagate_local<T, R>), dim3(1), dim3(T), lds, s, a, l, k, maxIter); \
+    if (owner)                                                                         \
+      hipLaunchKernelGGL((k_propagate_local<T, R, true>), dim3(1), dim3(T), lds, s, a, l, k, \
+                         maxIter);                                                     \
+    else                                                                               \
+      hipLaunchKernelGGL((k_propagate_local<T, R, false>), dim3(1), dim3(T), lds, s, a, l, \
+                         k, maxIter);                                                  \
     break;
     DLP_LOCAL_SHAPES(DLP_LAUNCH_LOCAL)
 #undef DLP_LAUNCH_LOCAL

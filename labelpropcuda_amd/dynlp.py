@@ -375,12 +375,12 @@ class Context:
         return b.value
 
     def set_tuning(self, groupLanes=0, blocks=0, chunk=0, localThreshold=0, localLdsBytes=0,
-                   localShape=0, localOwnerMode=0):
+                   localShape=0, localOwnerAfter=0):
         """0 keeps a setting; localThreshold < 0 disables the LDS-resident
-        propagation kernel; localShape is 1-based; localOwnerMode -1 runs the
-        LDS-resident kernel with queue iterations only."""
+        propagation kernel; localShape is 1-based; localOwnerAfter -1 runs
+        the LDS-resident kernel with queue iterations only."""
         t = _lib.Tuning(groupLanes, blocks, chunk, localThreshold, localLdsBytes, localShape,
-                        localOwnerMode)
+                        localOwnerAfter)
         check(self._lib.dlp_context_set_tuning(self._h, C.byref(t)))
 
     def tuning(self) -> dict:

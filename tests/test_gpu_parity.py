@@ -168,24 +168,30 @@ def test_engine_mix_long_rows(gpu_available, shape, owner):
     and in queue iterations."""
     inp = L.generate(3000, 40, 6, 7)
     prof = run_engine_mix(inp, 2000, localThreshold=1 << 30, localShape=shape,
-                          localOwnerMode=owner)
+                          localOwnerAfter=owner)
     assert prof["localLaunches"] > 0
     assert (prof["localOwnerUpdates"] > 0) == (owner > 0)
 
 
 @pytest.mark.parametrize("shape", [1, 2, 3, 4, 5])
 @pytest.mark.parametrize("owner,threshold,ldsBytes", [(1, 1 << 30, 0), (-1, 1 << 30, 0),
-                                                      (1, 64, 0), (1, 1 << 30, 8192)])
+                                                      (1, 64, 0), (1, 1 << 30, 8192),
+                                                      (5, 1 << 30, 0), (0, 1 << 30, 0)])
 def test_owner_mode_and_queue_iterations(gpu_available, c1_inputs, shape, owner, threshold,
                                          ldsBytes):
     """The LDS-resident kernel's owner mode (one thread per core vertex,
     frontier as flags) and its queue iterations leave the oracle's labels,
     iteration counts and frontier sizes for every work-group shape, with
-    frequent engine switches and with core rebuilds in a small LDS budget."""
+    owner mode from the first iteration of a pass or after some queue
+    iterations, frequent engine switches and core rebuilds in a small LDS
+    budget."""
     prof = run_engine_mix(c1_inputs, 3000, localThreshold=threshold, localLdsBytes=ldsBytes,
-                          localShape=shape, localOwnerMode=owner)
+                          localShape=shape, localOwnerAfter=owner)
     assert prof["localLaunches"] > 0
-    assert (prof["localOwnerUpdates"] > 0) == (owner > 0)
+    if owner < 0:
+        assert prof["localOwnerUpdates"] == 0
+    elif owner == 1:
+        assert prof["localOwnerUpdates"] > 0
     assert prof["localOwnerUpdates"] <= prof["localUpdates"]
 
 
