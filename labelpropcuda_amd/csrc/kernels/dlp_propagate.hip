@@ -761,7 +761,10 @@ __device__ __forceinline__ int local_passes(const PropArgs &a, const LocalArgs &
   const int P = l.ldsWords;
   const int stampBase = c->stampBase;
   int reason = 0;
-  bool useCore = true;
+  /* the owner kernel starts from the frontier alone: a long-lived frontier
+   * then grows the core (rebuilds) only by the vertices it reaches, which
+   * keeps it within one vertex per thread more often */
+  bool useCore = !OWNER;
   unsigned &profU = prof.U, &profE = prof.E, &profC = prof.C, &profA = prof.A;
   unsigned long long passUpdates = 0; /* tid 0: frontier sizes of this pass */
   const int kLaunch = k;
