@@ -25,6 +25,9 @@ struct PropArgs {
   float *lab1;
   int *iterStamp;     /* INT_MAX = may not enter a frontier                    */
   int *q0, *q1;       /* frontier queues                                       */
+  int queueCap;       /* entries each queue holds (0: unknown); lets the
+                         multi-block kernel load its first queue entry
+                         before it knows the frontier size                    */
   int *touched;       /* nullable: first write of a vertex in this batch       */
   DevCounters *ctr;
   int *iterSizes;     /* nullable: frontier size of every iteration            */

@@ -647,6 +647,7 @@ int dlp_ctx_run_propagation(dlp_context *c, int startK) {
 PropArgs dlp_ctx_prop_args(const dlp_context *c) {
   const DState &d = c->d;
   PropArgs a{};
+  a.queueCap = c->N;
   a.rowInfo = d.aRow;
   a.rowBeg = d.fRow;
   a.rowEnd = nullptr;

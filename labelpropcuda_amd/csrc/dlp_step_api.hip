@@ -307,6 +307,7 @@ extern "C" int dlp_iterative_propagation(const int *rowPtr, const int *colInd,
                                s)) != DLP_OK)
     return rc;
   PropArgs a{};
+  a.queueCap = numNodes;
   a.rowBeg = rowPtr;
   a.rowEnd = rowPtr + 1;
   a.rowLen = nullptr;

@@ -101,6 +101,18 @@ __global__ __launch_bounds__(kPT) void k_propagate(PropArgs a, int k) {
   __shared__ int stageT[kStageT];
   __shared__ int stageCount[2], stageBase[2];
   DevCounters *c = a.ctr;
+  const int *in = (k & 1) ? a.q1 : a.q0;
+  const int lane = threadIdx.x & 63;
+  const int gl = lane & (G - 1);
+  const int grp = lane / G;
+  const long long gwave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const long long nwaves = ((long long)gridDim.x * blockDim.x) >> 6;
+  /* the lane group's first queue entry, loaded together with the frontier
+   * size instead of after it (one dependent round trip less); used only if
+   * that entry is below the size */
+  const long long item0 = gwave * GPW + grp;
+  int u0 = 0;
+  if (item0 < a.queueCap) u0 = in[item0];
   const int n = c->qcount[k % 3];
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     c->qcount[(k + 2) % 3] = 0; /* consumed by iteration k-1; next iteration appends here */
@@ -117,17 +129,10 @@ __global__ __launch_bounds__(kPT) void k_propagate(PropArgs a, int k) {
 
   const int stampBase = c->stampBase;
   const int stampNext = stampBase + k + 1;
-  const int *in = (k & 1) ? a.q1 : a.q0;
   int *out = (k & 1) ? a.q0 : a.q1;
   int *outCount = &c->qcount[(k + 1) % 3];
   const float *Fold = (k & 1) ? a.lab1 : a.lab0;
   float *Fnew = (k & 1) ? a.lab0 : a.lab1;
-
-  const int lane = threadIdx.x & 63;
-  const int gl = lane & (G - 1);
-  const int grp = lane / G;
-  const long long gwave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const long long nwaves = ((long long)gridDim.x * blockDim.x) >> 6;
   unsigned profE = 0, profC = 0, profA = 0; /* lane-local work counters */
 
   for (long long base = gwave * GPW; base < n; base += nwaves * GPW) {  // wave-uniform
@@ -135,7 +140,7 @@ __global__ __launch_bounds__(kPT) void k_propagate(PropArgs a, int k) {
     const bool act = item < n;
     int u = 0, beg = 0, len = 0;
     if (act) {
-      u = in[item];
+      u = (item == item0 && a.queueCap > 0) ? u0 : in[item];
       if (a.rowInfo) {
         const int2 r = a.rowInfo[u];
         beg = r.x;
