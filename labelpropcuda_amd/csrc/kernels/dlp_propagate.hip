@@ -139,8 +139,10 @@ __global__ __launch_bounds__(kPT) void k_propagate(PropArgs a, int k) {
     const long long item = base + grp;
     const bool act = item < n;
     int u = 0, beg = 0, len = 0;
+    float oldLabel = 0.0f;
     if (act) {
       u = (item == item0 && a.queueCap > 0) ? u0 : in[item];
+      if (gl == 0) oldLabel = Fold[u];  // in flight with the row loads
       if (a.rowInfo) {
         const int2 r = a.rowInfo[u];
         beg = r.x;
@@ -155,6 +157,8 @@ __global__ __launch_bounds__(kPT) void k_propagate(PropArgs a, int k) {
      * group path needs no registers for the 256-slot form and the kernel
      * keeps a high occupancy */
     const bool longRow = act && len > 64;
+    /* the lane's first column, read by the sum below and kept for the marks */
+    const int col0 = (act && !longRow && gl < len) ? a.colInd[beg + gl] : -1;
     float W, S;
     dlp::group_short_row_sums<G>(gl, longRow ? 0 : len,
                                  GlobalPos{a.colInd, a.values, Fold, a.status, beg}, W, S);
@@ -175,7 +179,6 @@ __global__ __launch_bounds__(kPT) void k_propagate(PropArgs a, int k) {
     }
     int changed = 0;
     if (act && gl == 0) {
-      const float oldLabel = Fold[u];
       const float nl = dlp::new_label(W, S, oldLabel);
       Fnew[u] = nl;
       changed = fabsf(nl - oldLabel) > a.delta;
@@ -191,7 +194,7 @@ __global__ __launch_bounds__(kPT) void k_propagate(PropArgs a, int k) {
       int v = -1;
       bool pred = false;
       if (changed && p < len) {
-        v = a.colInd[beg + p];
+        v = (p0 == 0 && !longRow) ? col0 : a.colInd[beg + p];
         pred = v >= 0;
       } else if (changed && p == len) {
         v = u;
