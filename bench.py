@@ -159,6 +159,20 @@ def cpu_baseline(inputs, budget_s: float = 20.0, threads: int = 1):
     return upd / t, done, t
 
 
+def host_cpu() -> str:
+    """CPU model and logical CPU count of this host (the baseline's cores)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return f"{model}, {os.cpu_count()} logical CPUs visible"
+
+
 def cpu_baselines(inputs, runs: int = 3, budget_s: float = 10.0) -> dict:
     """The oracle on all host cores (at most 16, the box's share) as
     `cpu_baseline`, and single-threaded beside it: the median of `runs`
@@ -174,7 +188,7 @@ def cpu_baselines(inputs, runs: int = 3, budget_s: float = 10.0) -> dict:
                     "sample": f"median of {runs} runs, each over the first batches of the same "
                               f"stream until {budget_s:.0f} s are spent (median run: {done} of "
                               f"{inputs.numBatches} batches, {t:.1f} s), CPU oracle ({kind})",
-                    "runs": [r[0] for r in res]}
+                    "runs": [r[0] for r in res], "host": host_cpu()}
     return out
 
 
