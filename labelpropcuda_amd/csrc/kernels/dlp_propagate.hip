@@ -15,9 +15,10 @@
  *
  * Deduplication is an atomicMax on a per-vertex iteration stamp that also
  * carries eligibility (INT_MAX = may never enter a frontier).  Row sums use
- * dlp::group_row_sums, bit-identical to the reference's block reduction.
+ * dlp::group_row_sums / lane_row_sums, bit-identical to the reference's
+ * block reduction.
  *
- * Two kernels implement the same iteration:
+ * Kernels implementing the same iteration:
  *   k_propagate        one launch per iteration, many work-groups, G lanes
  *                      per frontier vertex, queues / labels / stamps in HBM;
  *                      for large frontiers.
@@ -26,7 +27,13 @@
  *                      of their neighbours staged in LDS; the limit cycles
  *                      and convergence tails of the reference's loop (most
  *                      iterations, with small frontiers) run without a
- *                      launch or a host round trip per iteration.
+ *                      launch or a host round trip per iteration.  Two
+ *                      variants: queue iterations (<..., false>) and owner
+ *                      mode (<..., true>: one thread per core vertex, the
+ *                      frontier as flags), which the host launches once a
+ *                      queue launch has run LocalArgs::ownerAfter iterations.
+ *   k_propagate_multi  one work-group per graph of a batched step (LDS
+ *                      passes and work-group HBM iterations).
  */
 #include "../dlp_kernels.h"
 #include "../dlp_rowsum.h"
