@@ -322,6 +322,25 @@ int dlp_context_set_profiling(dlp_context *ctx, int level);
 /* Profile accumulated since profiling was last turned on (synchronises). */
 int dlp_context_prop_profile(dlp_context *ctx, dlp_prop_profile *out);
 
+/* Wrap-around guards of the two device counters that only grow (no
+ * counterpart in the reference, whose loop restarts them per call).
+ * Propagation stamps iteration k of a batch stampBase + k + 1 and then
+ * advances stampBase by iterationsUsed + 1; the LDS-resident kernels add one
+ * to a generation counter per working set they build.  Before every batch
+ * the library checks upper bounds of both and, when the batch could reach
+ * the limit, rebases the stamps (eligible vertices' stamps to 0, stampBase
+ * to 1) or restarts the generations; results do not change.
+ * dlp_counter_guard_plan is that decision for given bounds (host only). */
+int dlp_counter_guard_plan(int64_t stampBound, int64_t genBound, int maxIterations,
+                           int *rebaseStamps, int *restartGens, int64_t *stampBoundAfter,
+                           int64_t *genBoundAfter);
+/* Moves a context's stampBase and generation counter up to the given
+ * values at a batch boundary (tests of the guards start near the limits);
+ * values may only grow. */
+int dlp_context_set_counter_bases(dlp_context *ctx, int stampBase, int localGen);
+/* Current device values of both counters (synchronises). */
+int dlp_context_counter_bases(dlp_context *ctx, int *stampBase, int *localGen);
+
 /* ====================================================================== */
 /* 4. Synthetic inputs (reference generateSparseGraph.cu / generateBatches.cu) */
 /* ====================================================================== */
@@ -393,7 +412,9 @@ int dlp_multi_destroy(dlp_multi *m);
 int dlp_multi_set_tuning(dlp_multi *m, const dlp_multi_tuning *t);
 int dlp_multi_get_tuning(const dlp_multi *m, dlp_multi_tuning *t);
 int dlp_multi_reset(dlp_multi *m);
-/* batch b of every graph that has one; batches run in order from a reset */
+/* batch b of every graph that has one; batches run in order from a reset
+ * (DLP_ERR_STATE after dlp_multi_run_next or dlp_multi_serve advanced
+ * graphs one by one, until dlp_multi_reset) */
 int dlp_multi_run_batch(dlp_multi *m, int b);
 /* reset, then every batch */
 int dlp_multi_run_all(dlp_multi *m);
@@ -404,6 +425,10 @@ int dlp_multi_read_labels(dlp_multi *m, int graph, float *labels);
 int dlp_multi_batch_stats(dlp_multi *m, int graph, int b, dlp_batch_stats *out);
 int dlp_multi_batch_time(dlp_multi *m, int b, float *ms);
 int dlp_multi_device_bytes(const dlp_multi *m, int64_t *bytes);
+
+/* dlp_context_set_counter_bases / dlp_context_counter_bases for graph g. */
+int dlp_multi_set_counter_bases(dlp_multi *m, int graph, int stampBase, int localGen);
+int dlp_multi_counter_bases(dlp_multi *m, int graph, int *stampBase, int *localGen);
 
 /* Serving.  Runs the next batch of each listed graph (distinct indices)
  * together: their pipelines, one propagation launch, their batch ends. */

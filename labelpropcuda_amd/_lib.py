@@ -268,6 +268,11 @@ SIGNATURES = {
     "dlp_device_free": (C.c_int, [_VP]),
     "dlp_copy_to_device": (C.c_int, [_VP, _VP, C.c_size_t]),
     "dlp_copy_to_host": (C.c_int, [_VP, _VP, C.c_size_t]),
+    "dlp_counter_guard_plan": (C.c_int, [C.c_int64, C.c_int64, C.c_int, i32p, i32p, i64p, i64p]),
+    "dlp_context_set_counter_bases": (C.c_int, [_VP, C.c_int, C.c_int]),
+    "dlp_context_counter_bases": (C.c_int, [_VP, i32p, i32p]),
+    "dlp_multi_set_counter_bases": (C.c_int, [_VP, C.c_int, C.c_int, C.c_int]),
+    "dlp_multi_counter_bases": (C.c_int, [_VP, C.c_int, i32p, i32p]),
 }
 
 _lib = None

@@ -40,6 +40,8 @@ struct dlp_context {
   LocalArgs local{};
   int localMaxBytes = 0;
   int nextBatch = 0;
+  /* upper bounds of the device's stampBase and localGen (dlp_guard_plan) */
+  long long stampBound = 1, genBound = 0;
   int markSeq = 0;
   int lastCCBatch = -1;
   std::vector<hipEvent_t> evStart, evEnd;
@@ -89,6 +91,11 @@ struct dlp_context {
   }
 };
 
+
+/* before a batch of the context: rebases its iteration stamps and restarts
+ * its LDS generations on stream s when the batch could wrap them (enqueued
+ * outside any captured graph: the decision is the host's) */
+int dlp_ctx_counter_guard(dlp_context *c, hipStream_t s);
 
 /* the batch pipeline of a context, in three phases (same stream) */
 DBatch dlp_ctx_make_batch(const dlp_context *c, int b);

@@ -252,6 +252,45 @@ struct DBatch {
   int nPrevGt;
 };
 
+/* ------------------------------------------------------------------------ */
+/* counters that grow across batches                                        */
+/* ------------------------------------------------------------------------ */
+/* Iteration stamps: batch b's propagation uses stamps stampBase + k + 1 for
+ * k < maxIterations and then advances stampBase by iterationsUsed + 1, and
+ * INT_MAX marks an ineligible vertex; the LDS kernels add 1 to localGen per
+ * working-set build.  Both only grow, so a long-lived context (a resident
+ * serving tenant) would wrap them.  The host keeps an upper bound of each
+ * and, before a batch that could cross the limit, enqueues a rebase:
+ * eligible stamps <- 0 with stampBase <- 1, or every core / halo generation
+ * <- 0 with localGen <- 0.  Both keep every invariant the kernels rely on
+ * (eligible stamps below stampBase between batches, generations below
+ * localGen). */
+constexpr int kMaxIterationsLimit = 1 << 28; /* larger maxIterations are rejected */
+constexpr long long kGenLimit = (1LL << 31) - 2;
+/* working-set builds of one batch, at most: a build either runs at least one
+ * iteration, ends the launch, or is retried once without the saved core */
+inline long long dlp_gens_per_batch(int maxIter) { return 4LL * ((long long)maxIter + 1) + 4; }
+struct CounterGuardPlan {
+  bool rebaseStamps, restartGens;
+  long long stampBound, genBound; /* bounds after the batch */
+};
+/* the guard decision before one batch, from the bounds before it */
+inline CounterGuardPlan dlp_guard_plan(long long stampBound, long long genBound,
+                                               int maxIter) {
+  CounterGuardPlan p{};
+  const long long perStamp = (long long)maxIter + 1;
+  p.rebaseStamps = stampBound + perStamp >= (long long)DLP_INT_MAX - 1;
+  p.stampBound = (p.rebaseStamps ? 1 : stampBound) + perStamp;
+  const long long perGen = dlp_gens_per_batch(maxIter);
+  p.restartGens = genBound + perGen >= kGenLimit;
+  p.genBound = (p.restartGens ? 0 : genBound) + perGen;
+  return p;
+}
+/* eligible stamps <- 0, ctr->stampBase <- 1 */
+int dlp_rebase_stamps(const DState &d, hipStream_t s);
+/* core and halo generations <- 0, ctr->localGen <- 0 */
+int dlp_restart_local_gens(const LocalArgs &l, DevCounters *ctr, hipStream_t s);
+
 int dlp_reset_state(const DState &d, hipStream_t s);
 int dlp_batch_begin(const DState &d, const DBatch &bt, hipStream_t s);
 int dlp_csr_update(const DState &d, const DBatch &bt, int markTag, hipStream_t s);

@@ -385,6 +385,7 @@ int capture(dlp_multi *m, hipGraphExec_t *out, Fn fn) {
 void host_reset(dlp_multi *m) {
   for (dlp_context *c : m->ctx) {
     c->nextBatch = 0;
+    c->stampBound = 1;  // the reset kernels set stampBase to 1
     c->lastCCBatch = -1;
     std::fill(c->ran.begin(), c->ran.end(), 0);
   }
@@ -564,12 +565,27 @@ extern "C" int dlp_multi_run_batch(dlp_multi *m, int b) {
     dlp_set_error_("dlp_multi_run_batch: NULL handle");
     return DLP_ERR_STATE;
   }
+  if (m->nextBatch < 0) {
+    dlp_set_error_("dlp_multi_run_batch: graphs were advanced one by one (run_next / serve); "
+                   "reset to run lockstep batches");
+    return DLP_ERR_STATE;
+  }
   if (b != m->nextBatch || b < 0 || b >= m->maxB) {
     dlp_set_errorf_("batch %d requested, next batch is %d (reset to start over)", b,
                     m->nextBatch);
     return DLP_ERR_STATE;
   }
+  for (int g = 0; g < m->G; ++g)
+    if (b < m->ctx[g]->B && m->ctx[g]->nextBatch != b) {
+      dlp_set_errorf_("dlp_multi_run_batch: graph %d is at batch %d, not %d", g,
+                      m->ctx[g]->nextBatch, b);
+      return DLP_ERR_STATE;
+    }
   int rc;
+  /* wrap-around guards: outside the captured graph, decided per run */
+  for (int g = 0; g < m->G; ++g)
+    if (b < m->ctx[g]->B && (rc = dlp_ctx_counter_guard(m->ctx[g], m->stream)) != DLP_OK)
+      return rc;
   bool justCaptured = false;
   if (m->useGraphs && !m->exec[b]) {
     /* the pipelines read host bookkeeping (mark tags, last CC batch) that
@@ -617,6 +633,27 @@ extern "C" int dlp_multi_run_batch(dlp_multi *m, int b) {
   m->ran[b] = 1;
   m->nextBatch = b + 1;
   return DLP_OK;
+}
+
+int dlp_ctx_set_counter_bases(dlp_context *c, int stampBase, int localGen);
+int dlp_ctx_counter_bases(dlp_context *c, int *stampBase, int *localGen);
+
+extern "C" int dlp_multi_set_counter_bases(dlp_multi *m, int g, int stampBase, int localGen) {
+  if (!m || g < 0 || g >= m->G) {
+    dlp_set_error_("dlp_multi_set_counter_bases: bad argument");
+    return DLP_ERR_INVALID;
+  }
+  DLP_HIP(hipStreamSynchronize(m->stream));
+  return dlp_ctx_set_counter_bases(m->ctx[g], stampBase, localGen);
+}
+
+extern "C" int dlp_multi_counter_bases(dlp_multi *m, int g, int *stampBase, int *localGen) {
+  if (!m || g < 0 || g >= m->G) {
+    dlp_set_error_("dlp_multi_counter_bases: bad argument");
+    return DLP_ERR_INVALID;
+  }
+  DLP_HIP(hipStreamSynchronize(m->stream));
+  return dlp_ctx_counter_bases(m->ctx[g], stampBase, localGen);
 }
 
 extern "C" int dlp_multi_run_all(dlp_multi *m) {
@@ -710,10 +747,12 @@ int run_next_impl(dlp_multi *m, const int *graphs, int count, int heavy,
                   const std::function<int(int, int)> *handOff = nullptr) {
   DLP_HIP(hipStreamSynchronize(m->stream));  // the staging buffer is free
   int rc;
+  m->nextBatch = -1;  // graphs advance one by one: no lockstep batch until a reset
   for (int i = 0; i < count; ++i) {
     dlp_context *c = m->ctx[graphs[i]];
     const int b = c->nextBatch;
     int tag = 0;
+    if ((rc = dlp_ctx_counter_guard(c, m->stream)) != DLP_OK) return rc;
     if ((rc = dlp_ctx_batch_pre(c, b, dlp_ctx_make_batch(c, b), &tag)) != DLP_OK) return rc;
     m->hStep[i] = m->props[graphs[i]];
     m->hStep[i].heavy = heavy;
@@ -810,6 +849,7 @@ extern "C" int dlp_multi_serve(dlp_multi *m, const dlp_serve_request *reqs, int 
       return DLP_ERR_INVALID;
     }
   }
+  m->nextBatch = -1;  // graphs advance one by one: no lockstep batch until a reset
   using clock = std::chrono::steady_clock;
   const clock::time_point t0 = clock::now();
   auto now = [&] { return std::chrono::duration<double>(clock::now() - t0).count(); };

@@ -87,11 +87,11 @@ int dlp_run_propagation(const PropArgs &a, const LocalArgs *l, int maxIter,
     const int n = pin->qcount[k % 3];
     if (n == 0) break;
     if (l && n <= cfg.localThreshold && k >= localAfter) {
-      if (pin->localGen > (1 << 30)) {  // restart the core generations
-        DLP_HIP(hipMemsetAsync(l->coreGen, 0, (size_t)l->numNodes * sizeof(int), s));
-        DLP_HIP(hipMemsetAsync(l->haloGen, 0, (size_t)l->numNodes * sizeof(int), s));
-        DLP_HIP(hipMemsetAsync(&a.ctr->localGen, 0, sizeof(int), s));
-      }
+      /* restart the core generations before a launch could carry localGen
+       * past kGenLimit (one launch builds at most dlp_gens_per_batch) */
+      if ((long long)pin->localGen + dlp_gens_per_batch(maxIter) >= kGenLimit &&
+          (rc = dlp_restart_local_gens(*l, a.ctr, s)) != DLP_OK)
+        return rc;
       if ((rc = begin_timed(timing, s, true, 1)) != DLP_OK) return rc;
       if ((rc = dlp_launch_propagate_local(a, *l, k, maxIter, owner, s)) != DLP_OK) return rc;
       if ((rc = end_timed(timing, s)) != DLP_OK) return rc;

@@ -152,8 +152,8 @@ int validate_inputs(const dlp_config *cfg, const dlp_graph_view *g, const float 
         return DLP_ERR_INVALID;
       }
   }
-  if (cfg->maxIterations < 0) {
-    dlp_set_error_("maxIterations must be >= 0");
+  if (cfg->maxIterations < 0 || cfg->maxIterations > kMaxIterationsLimit) {
+    dlp_set_errorf_("maxIterations must be in [0, %d]", kMaxIterationsLimit);
     return DLP_ERR_INVALID;
   }
   return DLP_OK;
@@ -570,6 +570,7 @@ extern "C" int dlp_context_reset(dlp_context *c) {
   }
   int rc = dlp_reset_state(c->d, c->stream);
   if (rc != DLP_OK) return rc;
+  c->stampBound = 1;  // k_reset sets stampBase to 1 and keeps localGen
   c->nextBatch = 0;
   c->lastCCBatch = -1;
   std::fill(c->ran.begin(), c->ran.end(), 0);
@@ -598,6 +599,7 @@ extern "C" int dlp_context_run_batch(dlp_context *c, int b) {
   hipStream_t s = c->stream;
   DBatch bt = dlp_ctx_make_batch(c, b);
   int rc, markTag = 0;
+  if ((rc = dlp_ctx_counter_guard(c, s)) != DLP_OK) return rc;
   DLP_HIP(hipEventRecord(c->evStart[b], s));
   if ((rc = dlp_ctx_batch_pre(c, b, bt, &markTag)) != DLP_OK) return rc;
   if (c->trace && (rc = trace_pre_propagation(c, bt, markTag)) != DLP_OK) return rc;
@@ -615,6 +617,79 @@ extern "C" int dlp_context_run_batch(dlp_context *c, int b) {
                         hipMemcpyDeviceToHost));
   }
   return DLP_OK;
+}
+
+int dlp_ctx_counter_guard(dlp_context *c, hipStream_t s) {
+  const CounterGuardPlan p =
+      dlp_guard_plan(c->stampBound, c->genBound, c->cfg.maxIterations);
+  int rc;
+  if (p.rebaseStamps && (rc = dlp_rebase_stamps(c->d, s)) != DLP_OK) return rc;
+  if (p.restartGens && (rc = dlp_restart_local_gens(c->local, c->d.ctr, s)) != DLP_OK) return rc;
+  c->stampBound = p.stampBound;
+  c->genBound = p.genBound;
+  return DLP_OK;
+}
+
+extern "C" int dlp_counter_guard_plan(int64_t stampBound, int64_t genBound, int maxIterations,
+                                        int *rebaseStamps, int *restartGens,
+                                        int64_t *stampBoundAfter, int64_t *genBoundAfter) {
+  if (!rebaseStamps || !restartGens || !stampBoundAfter || !genBoundAfter ||
+      maxIterations < 0 || maxIterations > kMaxIterationsLimit) {
+    dlp_set_error_("dlp_counter_guard_plan: bad argument");
+    return DLP_ERR_INVALID;
+  }
+  const CounterGuardPlan p = dlp_guard_plan(stampBound, genBound, maxIterations);
+  *rebaseStamps = p.rebaseStamps;
+  *restartGens = p.restartGens;
+  *stampBoundAfter = p.stampBound;
+  *genBoundAfter = p.genBound;
+  return DLP_OK;
+}
+
+int dlp_ctx_set_counter_bases(dlp_context *c, int stampBase, int localGen) {
+  if (stampBase < 1 || localGen < 0 || (long long)localGen >= kGenLimit) {
+    dlp_set_error_("counter bases: stampBase must be >= 1 and localGen in [0, 2^31 - 2)");
+    return DLP_ERR_INVALID;
+  }
+  DLP_HIP(hipStreamSynchronize(c->stream));
+  DevCounters ctr;
+  DLP_HIP(hipMemcpy(&ctr, c->d.ctr, sizeof(ctr), hipMemcpyDeviceToHost));
+  if (stampBase < ctr.stampBase || localGen < ctr.localGen) {
+    dlp_set_errorf_("counter bases may only grow (stampBase %d, localGen %d now)", ctr.stampBase,
+                    ctr.localGen);
+    return DLP_ERR_INVALID;
+  }
+  ctr.stampBase = stampBase;
+  ctr.localGen = localGen;
+  DLP_HIP(hipMemcpy(c->d.ctr, &ctr, sizeof(ctr), hipMemcpyHostToDevice));
+  c->stampBound = stampBase;
+  c->genBound = localGen;
+  return DLP_OK;
+}
+
+int dlp_ctx_counter_bases(dlp_context *c, int *stampBase, int *localGen) {
+  DLP_HIP(hipStreamSynchronize(c->stream));
+  DevCounters ctr;
+  DLP_HIP(hipMemcpy(&ctr, c->d.ctr, sizeof(ctr), hipMemcpyDeviceToHost));
+  if (stampBase) *stampBase = ctr.stampBase;
+  if (localGen) *localGen = ctr.localGen;
+  return DLP_OK;
+}
+
+extern "C" int dlp_context_set_counter_bases(dlp_context *c, int stampBase, int localGen) {
+  if (!c) {
+    dlp_set_error_("dlp_context_set_counter_bases: NULL context");
+    return DLP_ERR_STATE;
+  }
+  return dlp_ctx_set_counter_bases(c, stampBase, localGen);
+}
+
+extern "C" int dlp_context_counter_bases(dlp_context *c, int *stampBase, int *localGen) {
+  if (!c) {
+    dlp_set_error_("dlp_context_counter_bases: NULL context");
+    return DLP_ERR_STATE;
+  }
+  return dlp_ctx_counter_bases(c, stampBase, localGen);
 }
 
 int dlp_ctx_batch_pre(dlp_context *c, int b, const DBatch &bt, int *markTag) {

@@ -290,6 +290,10 @@ extern "C" int dlp_iterative_propagation(const int *rowPtr, const int *colInd,
     dlp_set_error_("dlp_iterative_propagation: bad argument");
     return DLP_ERR_INVALID;
   }
+  if (maxIterations > kMaxIterationsLimit) {  // iteration stamps are ints
+    dlp_set_errorf_("dlp_iterative_propagation: maxIterations above %d", kMaxIterationsLimit);
+    return DLP_ERR_INVALID;
+  }
   *iterationsUsed = 0;
   if (numNodes == 0) return DLP_OK;
   hipStream_t s = (hipStream_t)stream;

@@ -136,6 +136,24 @@ __global__ void k_reset(DState d) {
   }
 }
 
+/* wrap-around guards: eligible stamps (not INT_MAX) back to 0 under a
+ * stampBase of 1; every core / halo generation back to 0 with localGen */
+__global__ void k_rebase_stamps(int *iterStamp, long long n, DevCounters *ctr) {
+  const long long i = (long long)blockIdx.x * kT + threadIdx.x;
+  for (long long v = i; v < n; v += (long long)gridDim.x * kT)
+    if (iterStamp[v] != DLP_INT_MAX) iterStamp[v] = 0;
+  if (i == 0) ctr->stampBase = 1;
+}
+
+__global__ void k_restart_gens(int *coreGen, int *haloGen, long long n, DevCounters *ctr) {
+  const long long i = (long long)blockIdx.x * kT + threadIdx.x;
+  for (long long v = i; v < n; v += (long long)gridDim.x * kT) {
+    coreGen[v] = 0;
+    haloGen[v] = 0;
+  }
+  if (i == 0) ctr->localGen = 0;
+}
+
 /* zero per-batch counters; clear the previous batch's per-batch flags */
 __global__ void k_clear_prev(DState d, DBatch bt) {
   const int i = blockIdx.x * kT + threadIdx.x;
@@ -738,6 +756,20 @@ int grid_for(long long n) { return dlp_blocks(n, kT, 65536); }
 /* ========================================================================== */
 /* pipeline launchers                                                         */
 /* ========================================================================== */
+int dlp_rebase_stamps(const DState &d, hipStream_t s) {
+  hipLaunchKernelGGL(k_rebase_stamps, dim3(grid_for(d.N)), dim3(kT), 0, s, d.iterStamp,
+                     (long long)d.N, d.ctr);
+  DLP_LAUNCHED("k_rebase_stamps");
+  return DLP_OK;
+}
+
+int dlp_restart_local_gens(const LocalArgs &l, DevCounters *ctr, hipStream_t s) {
+  hipLaunchKernelGGL(k_restart_gens, dim3(grid_for(l.numNodes)), dim3(kT), 0, s, l.coreGen,
+                     l.haloGen, (long long)l.numNodes, ctr);
+  DLP_LAUNCHED("k_restart_gens");
+  return DLP_OK;
+}
+
 int dlp_reset_state(const DState &d, hipStream_t s) {
   hipLaunchKernelGGL(k_reset, dim3(grid_for(d.N)), dim3(kT), 0, s, d);
   DLP_LAUNCHED("k_reset");

@@ -430,6 +430,16 @@ class Context:
         check(self._lib.dlp_context_batch_stats(self._h, b, C.byref(s)))
         return {name: getattr(s, name) for name, _ in s._fields_}
 
+    def set_counter_bases(self, stampBase: int, localGen: int):
+        """Moves the iteration-stamp base and LDS generation counter up
+        (tests of the wrap-around guards)."""
+        check(self._lib.dlp_context_set_counter_bases(self._h, stampBase, localGen))
+
+    def counter_bases(self):
+        sb, lg = C.c_int(), C.c_int()
+        check(self._lib.dlp_context_counter_bases(self._h, C.byref(sb), C.byref(lg)))
+        return sb.value, lg.value
+
 
 class Multi:
     """Many independent graphs, one propagation launch per batch index
@@ -510,6 +520,14 @@ class Multi:
         check(self._lib.dlp_multi_device_bytes(self._h, C.byref(v)))
         return v.value
 
+    def set_counter_bases(self, g: int, stampBase: int, localGen: int):
+        check(self._lib.dlp_multi_set_counter_bases(self._h, g, stampBase, localGen))
+
+    def counter_bases(self, g: int):
+        sb, lg = C.c_int(), C.c_int()
+        check(self._lib.dlp_multi_counter_bases(self._h, g, C.byref(sb), C.byref(lg)))
+        return sb.value, lg.value
+
     def run_next(self, graphs):
         """The next batch of each listed graph, together."""
         arr = np.ascontiguousarray(graphs, np.int32)
@@ -528,6 +546,18 @@ class Multi:
         check(self._lib.dlp_multi_serve(self._h, reqs, n, C.byref(cfg),
                                         _ptr(done, C.c_double), C.byref(steps)))
         return done[:n], steps.value
+
+
+def counter_guard_plan(stampBound: int, genBound: int, maxIterations: int) -> dict:
+    """The library's wrap-around guard decision before one batch (host
+    arithmetic only)."""
+    lib = load()
+    rb, rg = C.c_int(), C.c_int()
+    sa, ga = C.c_int64(), C.c_int64()
+    check(lib.dlp_counter_guard_plan(stampBound, genBound, maxIterations, C.byref(rb),
+                                     C.byref(rg), C.byref(sa), C.byref(ga)))
+    return {"rebaseStamps": bool(rb.value), "restartGens": bool(rg.value),
+            "stampBound": sa.value, "genBound": ga.value}
 
 
 def batched_workload(numGraphs: int = 256, numBatches: int = 10, seed: int = 42,
