@@ -134,6 +134,24 @@ int dlp_iterative_propagation(const int *rowPtr, const int *colInd,
                               int numNodes, float delta, int maxIterations,
                               int *iterationsUsed, void *stream);
 
+/* dlp_iterative_propagation without any device allocation.  No counterpart
+ * in the reference: for callers that cannot allocate device memory (serving
+ * loops and harnesses that forbid hipMalloc).  Same arguments, semantics and
+ * results as dlp_iterative_propagation; its scratch (counters, stamps, the
+ * two frontier queues and the LDS-kernel bookkeeping: 28 bytes per vertex)
+ * lives in a static device array of 14.1 MiB per GPU, held until the
+ * process ends, so numNodes may be at most 524,288 (DLP_ERR_INVALID above).
+ * Every call on a GPU rewrites that GPU's one copy of the array (it keeps
+ * the last call's contents; each call clears what must start at zero), so
+ * two calls that overlapped on one GPU would race: the library serialises
+ * the calls of a process, and each returns with `stream` synchronised. */
+int dlp_iterative_propagation_noalloc(const int *rowPtr, const int *colInd,
+                                      const float *values, float *labels,
+                                      float *labelsBuffer, const int *vertexStatus,
+                                      const int *gtClass, int *isAffected,
+                                      int numNodes, float delta, int maxIterations,
+                                      int *iterationsUsed, void *stream);
+
 /* ====================================================================== */
 /* 2. runDynLP over host inputs                                           */
 /* ====================================================================== */

@@ -200,6 +200,9 @@ SIGNATURES = {
     "dlp_iterative_propagation": (
         C.c_int,
         [_VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP, C.c_int, C.c_float, C.c_int, i32p, _VP]),
+    "dlp_iterative_propagation_noalloc": (
+        C.c_int,
+        [_VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP, C.c_int, C.c_float, C.c_int, i32p, _VP]),
     "dlp_config_default": (None, [C.POINTER(Config)]),
     "dlp_run_dynlp": (
         C.c_int,

@@ -279,36 +279,32 @@ extern "C" int dlp_label_initialization(const int *rowPtr, const int *colInd,
   return DLP_OK;
 }
 
-extern "C" int dlp_iterative_propagation(const int *rowPtr, const int *colInd,
-                                         const float *values, float *labels,
-                                         float *labelsBuffer, const int *vertexStatus,
-                                         const int *gtClass, int *isAffected, int numNodes,
-                                         float delta, int maxIterations, int *iterationsUsed,
-                                         void *stream) {
-  if (!iterationsUsed || !rowPtr || !labels || !labelsBuffer || !vertexStatus || !gtClass ||
-      !isAffected || numNodes < 0) {
-    dlp_set_error_("dlp_iterative_propagation: bad argument");
-    return DLP_ERR_INVALID;
-  }
-  if (maxIterations > kMaxIterationsLimit) {  // iteration stamps are ints
-    dlp_set_errorf_("dlp_iterative_propagation: maxIterations above %d", kMaxIterationsLimit);
-    return DLP_ERR_INVALID;
-  }
-  *iterationsUsed = 0;
-  if (numNodes == 0) return DLP_OK;
-  hipStream_t s = (hipStream_t)stream;
-  std::lock_guard<std::mutex> lock(g_wsMutex);
-  Workspace *w;
-  int rc = ws_open(&w);
-  if (rc != DLP_OK) return rc;
-  if ((rc = ws_reset_counters(*w, s)) != DLP_OK) return rc;
-  WS_BUF(int, stamp, 1, numNodes);
-  WS_BUF(int, q0, 3, numNodes);
-  WS_BUF(int, q1, 18, numNodes);
+namespace {
+
+/* scratch of one propagation call: counters (device, and a host copy for the
+ * read-backs), stamps, the two queues and the LDS-kernel bookkeeping */
+struct PropScratch {
+  DevCounters *ctr;
+  DevCounters *host;
+  int *stamp, *q0, *q1;
+  int *coreGen; /* 2 * numNodes: core generations, then halo generations */
+  int *coreIdx, *haloIdx, *coreList, *ovf;
+  int listCap;
+  int ldsBytes;
+};
+
+/* launchIterativePropagation (iterativePropagation.cu:241-346) on the given
+ * scratch; the counters must hold zeros with stampBase 1 and the localGen
+ * the generations in coreGen were written under */
+int propagate_on(const PropScratch &ps, const int *rowPtr, const int *colInd, const float *values,
+                 float *labels, float *labelsBuffer, const int *vertexStatus, const int *gtClass,
+                 int *isAffected, int numNodes, float delta, int maxIterations,
+                 int *iterationsUsed, hipStream_t s) {
+  int rc;
   /* both buffers equal on entry (iterativePropagation.cu:268-270) */
   if ((rc = dlp_k_copy_float(labelsBuffer, labels, numNodes, s)) != DLP_OK) return rc;
-  if ((rc = dlp_k_prop_prepare(isAffected, vertexStatus, gtClass, numNodes, stamp, q0, w->ctr,
-                               s)) != DLP_OK)
+  if ((rc = dlp_k_prop_prepare(isAffected, vertexStatus, gtClass, numNodes, ps.stamp, ps.q0,
+                               ps.ctr, s)) != DLP_OK)
     return rc;
   PropArgs a{};
   a.queueCap = numNodes;
@@ -320,43 +316,28 @@ extern "C" int dlp_iterative_propagation(const int *rowPtr, const int *colInd,
   a.status = vertexStatus;
   a.lab0 = labels;
   a.lab1 = labelsBuffer;
-  a.iterStamp = stamp;
-  a.q0 = q0;
-  a.q1 = q1;
+  a.iterStamp = ps.stamp;
+  a.q0 = ps.q0;
+  a.q1 = ps.q1;
   a.touched = nullptr;
-  a.ctr = w->ctr;
+  a.ctr = ps.ctr;
   a.iterSizes = nullptr;
   a.iterSizesCap = 0;
   a.profile = 0;
   a.delta = delta;
 
   LocalArgs l{};
-  if (!w->ldsBytes && (rc = dlp_local_lds_bytes(&w->ldsBytes)) != DLP_OK) return rc;
-  l.ldsWords = w->ldsBytes / 4;
+  l.ldsWords = ps.ldsBytes / 4;
   l.numNodes = numNodes;
   l.shape = 0;
   l.ownerAfter = kOwnerAfterDefault;
-  l.coreListCap = l.ovfCap = dlp_local_list_cap(l.ldsWords);
-  WS_BUF(int, coreGen, 19, 2LL * numNodes);  // core generations, then halo generations
-  WS_BUF(int, coreIdx, 20, numNodes);
-  WS_BUF(int, haloIdx, 23, numNodes);
-  WS_BUF(int, coreList, 21, l.coreListCap);
-  WS_BUF(int, ovf, 22, l.ovfCap);
-  if (coreGen != w->coreGenBuf || w->sizes[19] != w->coreGenBytes) {
-    /* new or regrown buffer: restart the generations (the counters were
-     * already uploaded with the old localGen) */
-    DLP_HIP(hipMemsetAsync(coreGen, 0, w->sizes[19], s));
-    DLP_HIP(hipMemsetAsync(&w->ctr->localGen, 0, sizeof(int), s));
-    w->coreGenBuf = coreGen;
-    w->coreGenBytes = w->sizes[19];
-    w->localGen = 0;
-  }
-  l.coreGen = coreGen;
-  l.haloGen = coreGen + numNodes;
-  l.coreIdx = coreIdx;
-  l.haloIdx = haloIdx;
-  l.coreList = coreList;
-  l.ovf = ovf;
+  l.coreListCap = l.ovfCap = ps.listCap;
+  l.coreGen = ps.coreGen;
+  l.haloGen = ps.coreGen + numNodes;
+  l.coreIdx = ps.coreIdx;
+  l.haloIdx = ps.haloIdx;
+  l.coreList = ps.coreList;
+  l.ovf = ps.ovf;
   PropLoopConfig cfg;
   cfg.groupLanes = 8;
   cfg.blocks = dlp_auto_prop_blocks(numNodes);
@@ -368,23 +349,200 @@ extern "C" int dlp_iterative_propagation(const int *rowPtr, const int *colInd,
     const long long meanDeg = ((long long)std::max(nnz, 0) + numNodes - 1) / numNodes;
     cfg.localThreshold = dlp_local_default_threshold(l.ldsWords, meanDeg);
   }
-  rc = dlp_run_propagation(a, numNodes < (1 << 30) ? &l : nullptr, maxIterations, cfg, w->pin, s,
+  rc = dlp_run_propagation(a, numNodes < (1 << 30) ? &l : nullptr, maxIterations, cfg, ps.host, s,
                            nullptr);
   if (rc != DLP_OK) return rc;
-  if ((rc = ws_read_counters(*w, s)) != DLP_OK) return rc;
-  w->localGen = w->pin->localGen;
-  const int m = w->pin->iterUsed;
+  DLP_HIP(hipMemcpyAsync(ps.host, ps.ctr, sizeof(DevCounters), hipMemcpyDeviceToHost, s));
+  DLP_HIP(hipStreamSynchronize(s));
+  const int m = ps.host->iterUsed;
   *iterationsUsed = m;
   /* last written buffer ends in labels (iterativePropagation.cu:330-334) */
   if (m & 1) {
     if ((rc = dlp_k_copy_float(labels, labelsBuffer, numNodes, s)) != DLP_OK) return rc;
   }
-  const int capped = (m == maxIterations && m > 0 && w->pin->qcount[m % 3] > 0) ? 1 : 0;
+  const int capped = (m == maxIterations && m > 0 && ps.host->qcount[m % 3] > 0) ? 1 : 0;
   rc = dlp_k_prop_finish(labels, gtClass, numNodes, m > 0 ? isAffected : nullptr,
-                         (m & 1) ? q1 : q0, w->ctr, m % 3, capped, s);
+                         (m & 1) ? ps.q1 : ps.q0, ps.ctr, m % 3, capped, s);
   if (rc != DLP_OK) return rc;
   DLP_HIP(hipStreamSynchronize(s));
   return DLP_OK;
+}
+
+int check_propagation_args(const char *who, const int *rowPtr, const float *labels,
+                           const float *labelsBuffer, const int *vertexStatus, const int *gtClass,
+                           const int *isAffected, int numNodes, int maxIterations,
+                           const int *iterationsUsed) {
+  if (!iterationsUsed || !rowPtr || !labels || !labelsBuffer || !vertexStatus || !gtClass ||
+      !isAffected || numNodes < 0) {
+    dlp_set_errorf_("%s: bad argument", who);
+    return DLP_ERR_INVALID;
+  }
+  if (maxIterations > kMaxIterationsLimit) {  // iteration stamps are ints
+    dlp_set_errorf_("%s: maxIterations above %d", who, kMaxIterationsLimit);
+    return DLP_ERR_INVALID;
+  }
+  return DLP_OK;
+}
+
+}  // namespace
+
+extern "C" int dlp_iterative_propagation(const int *rowPtr, const int *colInd,
+                                         const float *values, float *labels,
+                                         float *labelsBuffer, const int *vertexStatus,
+                                         const int *gtClass, int *isAffected, int numNodes,
+                                         float delta, int maxIterations, int *iterationsUsed,
+                                         void *stream) {
+  int rc = check_propagation_args("dlp_iterative_propagation", rowPtr, labels, labelsBuffer,
+                                  vertexStatus, gtClass, isAffected, numNodes, maxIterations,
+                                  iterationsUsed);
+  if (rc != DLP_OK) return rc;
+  *iterationsUsed = 0;
+  if (numNodes == 0) return DLP_OK;
+  hipStream_t s = (hipStream_t)stream;
+  std::lock_guard<std::mutex> lock(g_wsMutex);
+  Workspace *w;
+  if ((rc = ws_open(&w)) != DLP_OK) return rc;
+  if ((rc = ws_reset_counters(*w, s)) != DLP_OK) return rc;
+  if (!w->ldsBytes && (rc = dlp_local_lds_bytes(&w->ldsBytes)) != DLP_OK) return rc;
+  const int listCap = dlp_local_list_cap(w->ldsBytes / 4);
+  WS_BUF(int, stamp, 1, numNodes);
+  WS_BUF(int, q0, 3, numNodes);
+  WS_BUF(int, q1, 18, numNodes);
+  WS_BUF(int, coreGen, 19, 2LL * numNodes);  // core generations, then halo generations
+  WS_BUF(int, coreIdx, 20, numNodes);
+  WS_BUF(int, haloIdx, 23, numNodes);
+  WS_BUF(int, coreList, 21, listCap);
+  WS_BUF(int, ovf, 22, listCap);
+  if (coreGen != w->coreGenBuf || w->sizes[19] != w->coreGenBytes) {
+    /* new or regrown buffer: restart the generations (the counters were
+     * already uploaded with the old localGen) */
+    DLP_HIP(hipMemsetAsync(coreGen, 0, w->sizes[19], s));
+    DLP_HIP(hipMemsetAsync(&w->ctr->localGen, 0, sizeof(int), s));
+    w->coreGenBuf = coreGen;
+    w->coreGenBytes = w->sizes[19];
+    w->localGen = 0;
+  }
+  const PropScratch ps{w->ctr, w->pin, stamp, q0, q1, coreGen, coreIdx, haloIdx,
+                       coreList, ovf, listCap, w->ldsBytes};
+  rc = propagate_on(ps, rowPtr, colInd, values, labels, labelsBuffer, vertexStatus, gtClass,
+                    isAffected, numNodes, delta, maxIterations, iterationsUsed, s);
+  w->localGen = w->pin->localGen;
+  return rc;
+}
+
+/* ------------------------------------------------------------------------ */
+/* propagation without allocation: scratch in a static device array         */
+/* ------------------------------------------------------------------------ */
+namespace {
+
+constexpr int kNoAllocMaxNodes = 1 << 19;   /* 524,288 vertices              */
+constexpr int kNoAllocListCap = 16384;      /* LDS-kernel core / overflow lists */
+constexpr long long kNoAllocCtrInts = 64;   /* 256 bytes for the counters    */
+constexpr long long kNoAllocInts =
+    kNoAllocCtrInts + 7LL * kNoAllocMaxNodes + 2LL * kNoAllocListCap; /* 14.1 MiB */
+static_assert(sizeof(DevCounters) <= 4 * kNoAllocCtrInts, "counters outgrow their slot");
+
+/* The scratch of dlp_iterative_propagation_noalloc, one copy per GPU, held
+ * until the process ends.  Host code never uses this name as an address:
+ * the per-device address comes from hipGetSymbolAddress (g_noAllocBase). */
+static __device__ __align__(256) int scratch[kNoAllocInts];
+
+constexpr int kNoAllocMaxDevices = 64;
+std::mutex g_noAllocMutex;
+int *g_noAllocBase[kNoAllocMaxDevices];  /* device address of `scratch` per GPU */
+int g_noAllocLds[kNoAllocMaxDevices];    /* LDS bytes of the local kernel per GPU */
+
+/* counters <- zero with stampBase 1 and localGen 0; generations <- 0 */
+__global__ void k_noalloc_clear(DevCounters *ctr, int *gens, long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  for (long long v = i; v < n; v += (long long)gridDim.x * blockDim.x) gens[v] = 0;
+  if (i == 0) {
+    DevCounters z = {};
+    z.stampBase = 1;
+    *ctr = z;
+  }
+}
+
+}  // namespace
+
+extern "C" int dlp_iterative_propagation_noalloc(const int *rowPtr, const int *colInd,
+                                                 const float *values, float *labels,
+                                                 float *labelsBuffer, const int *vertexStatus,
+                                                 const int *gtClass, int *isAffected,
+                                                 int numNodes, float delta, int maxIterations,
+                                                 int *iterationsUsed, void *stream) {
+  int rc = check_propagation_args("dlp_iterative_propagation_noalloc", rowPtr, labels,
+                                  labelsBuffer, vertexStatus, gtClass, isAffected, numNodes,
+                                  maxIterations, iterationsUsed);
+  if (rc != DLP_OK) return rc;
+  if (numNodes > kNoAllocMaxNodes) {
+    dlp_set_errorf_("dlp_iterative_propagation_noalloc: numNodes %d above the static scratch's "
+                    "limit of %d vertices",
+                    numNodes, kNoAllocMaxNodes);
+    return DLP_ERR_INVALID;
+  }
+  *iterationsUsed = 0;
+  if (numNodes == 0) return DLP_OK;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+    (void)hipGetLastError();
+    dlp_set_error_("no GPU visible to HIP");
+    return DLP_ERR_NODEVICE;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  std::lock_guard<std::mutex> lock(g_noAllocMutex);
+  int dev = 0;
+  DLP_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kNoAllocMaxDevices) {
+    dlp_set_errorf_("dlp_iterative_propagation_noalloc: device %d above %d", dev,
+                    kNoAllocMaxDevices - 1);
+    return DLP_ERR_INVALID;
+  }
+  if (!g_noAllocBase[dev]) {  // first call on this device: its copy's address
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      (void)hipGetLastError();
+      dlp_set_errorf_("hipGetSymbolAddress of the static scratch failed (%s)",
+                      hipGetErrorString(e));
+      return DLP_ERR_HIP;
+    }
+    if ((rc = dlp_local_lds_bytes(&g_noAllocLds[dev])) != DLP_OK) return rc;
+    g_noAllocBase[dev] = (int *)p;
+  }
+  const int listCap = dlp_local_list_cap(g_noAllocLds[dev] / 4);
+  if (listCap > kNoAllocListCap) {
+    dlp_set_errorf_("dlp_iterative_propagation_noalloc: LDS lists of %d entries above %d",
+                    listCap, kNoAllocListCap);
+    return DLP_ERR_INVALID;
+  }
+  int *base = g_noAllocBase[dev];
+  const long long N = numNodes;
+  PropScratch ps{};
+  ps.ctr = (DevCounters *)base;
+  ps.stamp = base + kNoAllocCtrInts;
+  ps.q0 = ps.stamp + N;
+  ps.q1 = ps.q0 + N;
+  ps.coreGen = ps.q1 + N;
+  ps.coreIdx = ps.coreGen + 2 * N;
+  ps.haloIdx = ps.coreIdx + N;
+  ps.coreList = ps.haloIdx + N;
+  ps.ovf = ps.coreList + listCap;
+  ps.listCap = listCap;
+  ps.ldsBytes = g_noAllocLds[dev];
+  if (ps.ovf + listCap > base + kNoAllocInts) {
+    dlp_set_error_("dlp_iterative_propagation_noalloc: scratch layout overflow");
+    return DLP_ERR_INTERNAL;
+  }
+  DevCounters host{};
+  ps.host = &host;
+  /* the array keeps the previous call's contents: counters and generations
+   * start from zero on every call */
+  const unsigned grid = (unsigned)std::min<long long>((2 * N + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_noalloc_clear, dim3(grid), dim3(256), 0, s, ps.ctr, ps.coreGen, 2 * N);
+  DLP_LAUNCHED("k_noalloc_clear");
+  return propagate_on(ps, rowPtr, colInd, values, labels, labelsBuffer, vertexStatus, gtClass,
+                      isAffected, numNodes, delta, maxIterations, iterationsUsed, s);
 }
 
 /* ------------------------------------------------------------------------ */
