@@ -187,7 +187,8 @@ def cpu_baselines(inputs, runs: int = 3, budget_s: float = 10.0) -> dict:
         out[key] = {"value": v, "unit": "updates/s", "cores": th, "kind": "port",
                     "sample": f"median of {runs} runs, each over the first batches of the same "
                               f"stream until {budget_s:.0f} s are spent (median run: {done} of "
-                              f"{inputs.numBatches} batches, {t:.1f} s), CPU oracle ({kind})",
+                              f"{inputs.numBatches} batches, {t:.1f} s), CPU oracle ({kind}, "
+                              "gcc -O3 -ffp-contract=off)",
                     "runs": [r[0] for r in res], "host": host_cpu()}
     return out
 

@@ -667,12 +667,14 @@ __global__ __launch_bounds__(kT) void k_export_rows(DState d, const int *rowPtr,
                                                     float *values) {
   constexpr int G = 8;
   const int lane = threadIdx.x & 63, gl = lane & (G - 1), grp = lane / G;
-  const long long u = (((long long)blockIdx.x * kT + threadIdx.x) >> 6) * (64 / G) + grp;
-  if (u >= d.N) return;
-  const int src = d.fRow[u], dst = rowPtr[u], len = d.aLen[u];
-  for (int p = gl; p < len; p += G) {
-    colInd[dst + p] = d.aCol[src + p];
-    values[dst + p] = d.aVal[src + p];
+  const long long gwave = ((long long)blockIdx.x * kT + threadIdx.x) >> 6;
+  const long long nwaves = ((long long)gridDim.x * kT) >> 6;
+  for (long long u = gwave * (64 / G) + grp; u < d.N; u += nwaves * (64 / G)) {
+    const int src = d.fRow[u], dst = rowPtr[u], len = d.aLen[u];
+    for (int p = gl; p < len; p += G) {
+      colInd[dst + p] = d.aCol[src + p];
+      values[dst + p] = d.aVal[src + p];
+    }
   }
 }
 
@@ -705,12 +707,14 @@ __global__ __launch_bounds__(kT) void k_md_clean(const int *rowPtr, int *colInd,
                                                  const int *status, int numNodes) {
   constexpr int G = 8;
   const int lane = threadIdx.x & 63, gl = lane & (G - 1), grp = lane / G;
-  const long long u = (((long long)blockIdx.x * kT + threadIdx.x) >> 6) * (64 / G) + grp;
-  if (u >= numNodes) return;
-  if (status[u] == DLP_VERTEX_DELETED) return;
-  for (int e = rowPtr[u] + gl; e < rowPtr[u + 1]; e += G) {
-    int v = colInd[e];
-    if (v >= 0 && v < numNodes && status[v] == DLP_VERTEX_DELETED) colInd[e] = -1;
+  const long long gwave = ((long long)blockIdx.x * kT + threadIdx.x) >> 6;
+  const long long nwaves = ((long long)gridDim.x * kT) >> 6;
+  for (long long u = gwave * (64 / G) + grp; u < numNodes; u += nwaves * (64 / G)) {
+    if (status[u] == DLP_VERTEX_DELETED) continue;
+    for (int e = rowPtr[u] + gl; e < rowPtr[u + 1]; e += G) {
+      int v = colInd[e];
+      if (v >= 0 && v < numNodes && status[v] == DLP_VERTEX_DELETED) colInd[e] = -1;
+    }
   }
 }
 
