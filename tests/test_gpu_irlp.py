@@ -9,6 +9,7 @@ import pytest
 
 import labelpropcuda_amd as L
 import oracle.oracle as O
+from oracle.oracle import Oracle
 
 pytestmark = pytest.mark.gpu
 F32 = np.float32
@@ -113,3 +114,24 @@ def test_cli_validate_writes_irlp_labels(gpu_available, tmp_path):
     assert rows.shape == (4000, 2)
     assert np.array_equal(rows[:, 0].astype(int), np.arange(4000))
     assert np.array_equal(rows[:, 1], np.array([float(f"{x:.6f}") for x in exp]))
+    # the DynLP outputs against the oracle: labels file text exactly, the
+    # accuracy column of results.csv exactly, times with two decimals
+    orc = Oracle(inp)
+    accs = []
+    for b in range(inp.numBatches):
+        orc.run_batch(b)
+        accs.append(orc.stats()["accuracy"])
+    want = "".join(f"{i} {float(x):.6f}\n" for i, x in enumerate(orc.labels()))
+    assert (tmp_path / "dynlp_labels.txt").read_text() == want
+    csv = (tmp_path / "results.csv").read_text().splitlines()
+    assert csv[0] == "batch,time_ms,accuracy" and len(csv) == 1 + inp.numBatches
+    for b, line in enumerate(csv[1:]):
+        idx, ms, acc = line.split(",")
+        assert int(idx) == b and acc == f"{float(np.float32(accs[b])):.6f}", line
+        assert len(ms.split(".")[1]) == 2 and float(ms) >= 0.0, line
+    for line in ("  dynlp_labels.txt  \u2014 Final predicted labels",
+                 "  results.csv       \u2014 Per-batch timing and accuracy",
+                 "  irlp_labels.txt   \u2014 IrLP baseline labels",
+                 "  graph/            \u2014 Generated graph in CSR format"):
+        assert line in r.stdout.splitlines(), line
+    assert "Total DynLP time:" in r.stdout and "Per-batch results:" in r.stdout
