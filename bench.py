@@ -1,4 +1,5 @@
-"""Benchmark: affected-vertex updates/s of the plain one-graph GPU loop.
+"""Benchmark: affected-vertex updates/s of the plain one-graph GPU loop,
+with the batched and served forms against the plain loop beside it.
 
 Workload (one "step"): the full 10-batch DynLP update stream of a
 500,000-vertex, average-degree-7 synthetic graph (generator seed 42, batch
@@ -10,6 +11,14 @@ fills) is part of each step.
 value = U / step time, U = sum over batches and iterations of the frontier
 size (the reference's numAffected), identical for every implementation that
 matches the reference's results.
+
+The default line also carries two comparison legs (skip with --no-batched /
+--no-serving): "batched" -- 256 graphs (2,000-200,000 vertices, degree
+3/5/7, 10 batches each) through the batched runner against the plain loop
+over the same graphs, labels compared bit for bit; "serving" -- 64 resident
+tenants (10,000-5,000,000 vertices) and 128 update batches all arriving at
+t = 0, served with coalesced steps against one batch at a time through the
+plain path, every tenant's labels compared bit for bit.
 
 Prints ONE JSON line.  --gpus N with N > 1 exits: this build runs on one GPU.
 """
@@ -179,13 +188,13 @@ def cpu_baselines(inputs, runs: int = 3, budget_s: float = 10.0) -> dict:
     runs, each over the stream's first batches until `budget_s` is spent."""
     cores = max(1, min(16, os.cpu_count() or 1))
     out = {}
-    for key, th in (("cpu_baseline", cores), ("cpu_baseline_1thread", 1)):
-        res = sorted((cpu_baseline(inputs, budget_s, threads=th) for _ in range(runs)),
+    for key, th, n in (("cpu_baseline", cores, runs), ("cpu_baseline_1thread", 1, 1)):
+        res = sorted((cpu_baseline(inputs, budget_s, threads=th) for _ in range(n)),
                      key=lambda r: r[0])
         v, done, t = res[len(res) // 2]
         kind = "frontier-parallel, OpenMP" if th > 1 else "single thread"
         out[key] = {"value": v, "unit": "updates/s", "cores": th, "kind": "port",
-                    "sample": f"median of {runs} runs, each over the first batches of the same "
+                    "sample": f"median of {n} runs, each over the first batches of the same "
                               f"stream until {budget_s:.0f} s are spent (median run: {done} of "
                               f"{inputs.numBatches} batches, {t:.1f} s), CPU oracle ({kind}, "
                               "gcc -O3 -ffp-contract=off)",
@@ -193,20 +202,17 @@ def cpu_baselines(inputs, runs: int = 3, budget_s: float = 10.0) -> dict:
     return out
 
 
-def batched_main(args) -> int:
-    """Batched form: many independent evolving graphs, one propagation
-    launch per batch index (HIP-graph replay), against the plain loop over
-    the same graphs one at a time."""
+def batched_leg(args, progress, autotune: bool) -> dict:
+    """The batched workload (256 graphs, vertices log-uniform in [2,000,
+    200,000], degree in {3,5,7}, 10 batches each): the plain loop, one graph
+    at a time, each graph's full stream (one pass, timed), then the batched
+    runner (one propagation launch per batch index, HIP-graph replay) over
+    the same graphs; labels compared bit for bit."""
     import labelpropcuda_amd as L
     t0 = time.perf_counter()
     graphs = L.batched_workload(args.graphs, args.batches, 42)
     gen_s = time.perf_counter() - t0
-
-    def progress(msg):
-        print(f"[bench batched {time.perf_counter() - t0:7.1f} s] {msg}", file=sys.stderr,
-              flush=True)
-
-    progress(f"generated {len(graphs)} graphs")
+    progress(f"generated {len(graphs)} graphs in {gen_s:.1f} s")
     # plain loop: one graph at a time, each with its own context (the first
     # graph is run once untimed to load the kernels)
     plain_ms, plain_upd, plain_lat, plain_labels = 0.0, 0, [], []
@@ -219,7 +225,7 @@ def batched_main(args) -> int:
             ctx.run_all()
             ctx.synchronize()
             plain_ms += 1000.0 * (time.perf_counter() - t1)
-            if gi % 16 == 15:
+            if gi % 32 == 31:
                 progress(f"plain loop: {gi + 1} graphs, {plain_ms / 1000.0:.1f} s of updates")
             st = [ctx.stats(b) for b in range(inp.numBatches)]
             plain_upd += sum(x["updates"] for x in st)
@@ -228,34 +234,36 @@ def batched_main(args) -> int:
 
     m = L.Multi(graphs)
     progress("multi-graph runner created")
-    # autotuner: work-group shape x hand-back frontier, one step each; a
-    # setting whose capturing step is already 1.5x the best so far is dropped
     trials = []
-    for shape in (3, 1):
-        for heavy in args.heavy_sweep:
-            m.set_tuning(localShape=shape, heavyFrontier=heavy, useGraphs=1)
-            t1 = time.perf_counter()
-            m.run_all()  # captures the batch graphs
-            m.synchronize()
-            first = time.perf_counter() - t1
-            progress(f"autotuner: shape {shape} hand-back {heavy}: capturing step "
-                     f"{1000.0 * first:.1f} ms")
-            if trials and first > 1.5 * min(trials)[0]:
-                continue
-            t1 = time.perf_counter()
-            m.run_all()
-            m.synchronize()
-            trials.append((time.perf_counter() - t1, shape, heavy))
-            progress(f"autotuner: shape {shape} hand-back {heavy}: "
-                     f"{1000.0 * trials[-1][0]:.1f} ms")
-    best = min(trials)
-    m.set_tuning(localShape=best[1], heavyFrontier=best[2], useGraphs=1)
-    for _ in range(max(args.warmup, 1)):
-        m.run_all()
+    if autotune:
+        # work-group shape x hand-back frontier, one step each; a setting
+        # whose capturing step is already 1.5x the best so far is dropped
+        for shape in (3, 1):
+            for heavy in args.heavy_sweep:
+                m.set_tuning(localShape=shape, heavyFrontier=heavy, useGraphs=1)
+                t1 = time.perf_counter()
+                m.run_all()  # captures the batch graphs
+                m.synchronize()
+                first = time.perf_counter() - t1
+                progress(f"autotuner: shape {shape} hand-back {heavy}: capturing step "
+                         f"{1000.0 * first:.1f} ms")
+                if trials and first > 1.5 * min(trials)[0]:
+                    continue
+                t1 = time.perf_counter()
+                m.run_all()
+                m.synchronize()
+                trials.append((time.perf_counter() - t1, shape, heavy))
+                progress(f"autotuner: shape {shape} hand-back {heavy}: "
+                         f"{1000.0 * trials[-1][0]:.1f} ms")
+        best = min(trials)
+        m.set_tuning(localShape=best[1], heavyFrontier=best[2], useGraphs=1)
+    for _ in range(max(args.batched_warmup, 1)):
+        m.run_all()  # the first step captures the batch graphs
     m.synchronize()
+    progress("batched runner warmed up")
     lat, upd = [], None
     t1 = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(args.batched_steps):
         m.run_all()
         m.synchronize()
         u = sum(m.stats(g, b)["updates"] for g, inp in enumerate(graphs)
@@ -269,26 +277,17 @@ def batched_main(args) -> int:
     tuning = m.tuning()
     m.close()
     assert upd == plain_upd, "batched and plain update counts differ"
-    ms_per_step = 1000.0 * wall / args.steps
-    value = upd / (wall / args.steps)
+    value = upd / (wall / args.batched_steps)
     plain_value = plain_upd / (plain_ms / 1000.0)
+    progress(f"batched {value / 1e6:.1f} M updates/s, plain {plain_value / 1e6:.1f} M, "
+             f"labels identical: {same}")
     out = {
-        "metric": f"affected-vertex updates/s (batched, {len(graphs)} graphs)",
-        "value": value,
-        "unit": "updates/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic (reference generator, graph g seed 42+g)",
-        "config": {"workload": f"{len(graphs)} graphs, vertices log-uniform in [2000, 200000], "
-                               f"degree in {{3,5,7}}, {args.batches} batches (90/1/9) each",
-                   "graphs": len(graphs), "vertices_total": int(sum(g.numNodes for g in graphs)),
-                   "updates_per_step": upd, "generation_s": gen_s},
+        "workload": f"{len(graphs)} graphs, vertices log-uniform in [2000, 200000], "
+                    f"degree in {{3,5,7}}, {args.batches} batches (90/1/9) each, graph g seed 42+g",
+        "graphs": len(graphs), "vertices_total": int(sum(g.numNodes for g in graphs)),
+        "updates_per_step": upd, "generation_s": gen_s,
+        "value": value, "unit": "updates/s", "steps": args.batched_steps,
+        "ms_per_step": 1000.0 * wall / args.batched_steps,
         "batch_latency_ms": {"p50": float(np.percentile(lat, 50)),
                              "p99": float(np.percentile(lat, 99))},
         "plain_loop": {"value": plain_value, "ms_per_step": plain_ms,
@@ -296,26 +295,58 @@ def batched_main(args) -> int:
                                             "p99": float(np.percentile(plain_lat, 99))}},
         "speedup_vs_plain": value / plain_value,
         "labels_identical_to_plain": bool(same),
-        "autotuner": [{"shape": s_, "heavyFrontier": h_, "ms": 1000.0 * t_}
-                      for t_, s_, h_ in trials],
         "tuning": tuning,
+    }
+    if trials:
+        out["autotuner"] = [{"shape": s_, "heavyFrontier": h_, "ms": 1000.0 * t_}
+                            for t_, s_, h_ in trials]
+    return out
+
+
+def batched_main(args) -> int:
+    """Batched form on its own line (with the in-bench shape sweep)."""
+    t0 = time.perf_counter()
+
+    def progress(msg):
+        print(f"[bench batched {time.perf_counter() - t0:7.1f} s] {msg}", file=sys.stderr,
+              flush=True)
+
+    args.batched_steps = args.steps
+    args.batched_warmup = args.warmup
+    leg = batched_leg(args, progress, autotune=True)
+    out = {
+        "metric": f"affected-vertex updates/s (batched, {leg['graphs']} graphs)",
+        "value": leg["value"],
+        "unit": "updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": leg["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (reference generator, graph g seed 42+g)",
+        "config": {"workload": leg["workload"], "graphs": leg["graphs"],
+                   "vertices_total": leg["vertices_total"],
+                   "updates_per_step": leg["updates_per_step"],
+                   "generation_s": leg["generation_s"]},
+        "batch_latency_ms": leg["batch_latency_ms"],
+        "plain_loop": leg["plain_loop"],
+        "speedup_vs_plain": leg["speedup_vs_plain"],
+        "labels_identical_to_plain": leg["labels_identical_to_plain"],
+        "autotuner": leg.get("autotuner"),
+        "tuning": leg["tuning"],
     }
     print(json.dumps(out))
     return 0
 
 
-def serving_main(args) -> int:
-    """Serving form: tenant graphs resident in HBM, update batches arriving
-    as a Poisson process, admitted per step by slots and HBM budget and run
-    together, against the same arrivals served one batch at a time with the
-    plain single-graph path."""
+def serving_setup(args, progress):
+    """The serving workload: tenant graphs (vertices log-uniform in [10,000,
+    max], degree 5, tenant t seed 1042+t) and the requests (tenant, size
+    log-uniform in [100, 100,000]), from numpy.random.default_rng(1042)."""
     import labelpropcuda_amd as L
-    t0 = time.perf_counter()
-
-    def progress(msg):
-        print(f"[bench serving {time.perf_counter() - t0:7.1f} s] {msg}", file=sys.stderr,
-              flush=True)
-
     rng = np.random.default_rng(1042)
     T, R = args.tenants, args.requests
     sizesN = np.rint(np.exp(rng.uniform(np.log(1e4), np.log(args.max_tenant_nodes), T)))
@@ -325,83 +356,133 @@ def serving_main(args) -> int:
     for t in range(T):
         sizes = [0] + [int(k) for k in reqSize[reqTenant == t]]
         tenants.append(L.generate_stream(int(sizesN[t]), 5, sizes, 1042 + t))
-        if t % 8 == 7:
+        if t % 16 == 15:
             progress(f"generated {t + 1} tenants")
     progress(f"generated {T} tenants, {int(sizesN.sum())} vertices, {R} requests")
+    return rng, tenants, reqTenant
+
+
+def serve_run(m, tenants, reqTenant, arrivals, plain, args, keep_labels=False):
+    """One pass of the requests through dlp_multi_serve from the tenants'
+    seed state; returns throughput and per-request latency (and the labels
+    of every tenant when asked)."""
+    T = len(tenants)
+    m.reset()
+    m.run_next(list(range(T)))  # seed batches
+    m.synchronize()
+    reqs = list(zip(reqTenant.tolist(), arrivals.tolist()))
+    done, steps = m.serve(reqs, maxSlots=args.slots, hbmBudgetBytes=args.hbm_budget,
+                          plain=plain, handBackLanes=args.lanes_back)
+    m.synchronize()
+    lat = 1000.0 * (done - arrivals)
+    assert np.all(lat >= 0.0), "a request completed before it arrived"
+    upd = sum(m.stats(t, b)["updates"] for t in range(T)
+              for b in range(1, tenants[t].numBatches))
+    out = {"makespan_s": float(done.max()), "steps": steps, "updates": int(upd),
+           "updates_per_s": upd / float(done.max()),
+           "min_ms": float(lat.min()), "p50_ms": float(np.percentile(lat, 50)),
+           "p99_ms": float(np.percentile(lat, 99))}
+    if keep_labels:
+        out["_labels"] = [m.labels(t) for t in range(T)]
+    return out
+
+
+def median_serve(m, tenants, reqTenant, arrivals, plain, args, progress, label):
+    """Several saturated passes (the makespan is about a second): the one of
+    median throughput, with every pass's rate, and the labels of the last."""
+    runs = []
+    for i in range(max(1, args.repeats)):
+        runs.append(serve_run(m, tenants, reqTenant, arrivals, plain, args,
+                              keep_labels=(i == max(1, args.repeats) - 1)))
+        progress(f"{label}, saturated, run {i + 1}: {runs[-1]['makespan_s']:.2f} s")
+    assert len({r["updates"] for r in runs}) == 1
+    labels = runs[-1].pop("_labels")
+    order = sorted(range(len(runs)), key=lambda i: runs[i]["updates_per_s"])
+    med = dict(runs[order[len(runs) // 2]])
+    med["runs_updates_per_s"] = [r["updates_per_s"] for r in runs]
+    return med, labels
+
+
+def serving_leg(args, progress, sweep_rates) -> dict:
+    """Saturated serving (all requests at t = 0) against the plain path
+    (one batch at a time through the single-graph loop), labels of every
+    tenant compared bit for bit; optionally a sweep of Poisson arrival
+    rates."""
+    import labelpropcuda_amd as L
+    rng, tenants, reqTenant = serving_setup(args, progress)
+    T, R = len(tenants), len(reqTenant)
     m = L.Multi(tenants)
     if args.heavy_frontier:
         m.set_tuning(heavyFrontier=args.heavy_frontier)
     progress("tenants resident")
-
-    def run(arrivals, plain):
-        m.reset()
-        m.run_next(list(range(T)))  # seed batches
-        m.synchronize()
-        reqs = list(zip(reqTenant.tolist(), arrivals.tolist()))
-        done, steps = m.serve(reqs, maxSlots=args.slots, hbmBudgetBytes=args.hbm_budget,
-                              plain=plain, handBackLanes=args.lanes_back)
-        m.synchronize()
-        lat = 1000.0 * (done - arrivals)
-        assert np.all(lat >= 0.0), "a request completed before it arrived"
-        upd = sum(m.stats(t, b)["updates"] for t in range(T)
-                  for b in range(1, tenants[t].numBatches))
-        return {"makespan_s": float(done.max()), "steps": steps, "updates": int(upd),
-                "updates_per_s": upd / float(done.max()),
-                "min_ms": float(lat.min()), "p50_ms": float(np.percentile(lat, 50)),
-                "p99_ms": float(np.percentile(lat, 99))}
-
-    def median_run(arrivals, plain, label):
-        # the saturated makespan is about a second: the run of median
-        # throughput out of several, with every run's rate reported
-        runs = []
-        for i in range(max(1, args.repeats)):
-            runs.append(run(arrivals, plain))
-            progress(f"{label}, saturated, run {i + 1}: {runs[-1]['makespan_s']:.2f} s")
-        assert len({r["updates"] for r in runs}) == 1
-        order = sorted(range(len(runs)), key=lambda i: runs[i]["updates_per_s"])
-        med = dict(runs[order[len(runs) // 2]])
-        med["runs_updates_per_s"] = [r["updates_per_s"] for r in runs]
-        return med
-
     zero = np.zeros(R)
-    sat_plain = median_run(zero, True, "plain")
-    sat_batched = median_run(zero, False, "batched")
-    assert sat_plain["updates"] == sat_batched["updates"]
+    sat_plain, lab_plain = median_serve(m, tenants, reqTenant, zero, True, args, progress,
+                                        "plain")
+    sat_served, lab_served = median_serve(m, tenants, reqTenant, zero, False, args, progress,
+                                          "served")
+    assert sat_plain["updates"] == sat_served["updates"]
+    same = all(np.array_equal(a.view(np.int32), b.view(np.int32))
+               for a, b in zip(lab_plain, lab_served))
+    progress(f"served {sat_served['updates_per_s'] / 1e6:.1f} M updates/s, plain "
+             f"{sat_plain['updates_per_s'] / 1e6:.1f} M, labels identical: {same}")
     service = sat_plain["makespan_s"] / R  # mean plain service time
     sweep = []
-    for f in args.rates:
+    for f in sweep_rates:
         gaps = rng.exponential(service / f, R)
         arrivals = np.cumsum(gaps) - gaps[0]
         row = {"rate_x_plain_capacity": f, "arrivals_per_s": f / service,
-               "batched": run(arrivals, False), "plain": run(arrivals, True)}
+               "batched": serve_run(m, tenants, reqTenant, arrivals, False, args),
+               "plain": serve_run(m, tenants, reqTenant, arrivals, True, args)}
         sweep.append(row)
         progress(f"rate {f} x capacity: batched p99 {row['batched']['p99_ms']:.1f} ms, "
                  f"plain p99 {row['plain']['p99_ms']:.1f} ms")
     tuning = m.tuning()
     m.close()
-    value = sat_batched["updates_per_s"]
     out = {
-        "metric": f"affected-vertex updates/s (serving, {T} tenants, saturated)",
-        "value": value,
+        "workload": f"{T} tenants, vertices log-uniform in [10000, "
+                    f"{int(args.max_tenant_nodes)}], degree 5, tenant t seed 1042+t; {R} "
+                    "batches of log-uniform 100-100000 vertices (90/1/9)",
+        "slots": args.slots, "hbm_budget_bytes": args.hbm_budget,
+        "updates": sat_served["updates"],
+        "saturated": {"served": sat_served, "plain": sat_plain,
+                      "speedup_vs_plain": sat_served["updates_per_s"] /
+                      sat_plain["updates_per_s"],
+                      "labels_identical_to_plain": bool(same)},
+        "tuning": tuning,
+    }
+    if sweep:
+        out["rate_sweep"] = sweep
+    return out
+
+
+def serving_main(args) -> int:
+    """Serving form on its own line, with the arrival-rate sweep."""
+    t0 = time.perf_counter()
+
+    def progress(msg):
+        print(f"[bench serving {time.perf_counter() - t0:7.1f} s] {msg}", file=sys.stderr,
+              flush=True)
+
+    leg = serving_leg(args, progress, args.rates)
+    sat = leg["saturated"]
+    out = {
+        "metric": f"affected-vertex updates/s (serving, {args.tenants} tenants, saturated)",
+        "value": sat["served"]["updates_per_s"],
         "unit": "updates/s",
         "n_gpus": 1,
-        "steps": sat_batched["steps"],
+        "steps": sat["served"]["steps"],
         "warmup": 0,
-        "ms_per_step": 1000.0 * sat_batched["makespan_s"] / max(sat_batched["steps"], 1),
+        "ms_per_step": 1000.0 * sat["served"]["makespan_s"] / max(sat["served"]["steps"], 1),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (reference graph generator, tenant t seed 1042+t, streamed batches)",
-        "config": {"workload": f"{T} tenants, vertices log-uniform in [10000, "
-                               f"{int(args.max_tenant_nodes)}], degree 5; {R} batches of "
-                               "log-uniform 100-100000 vertices (90/1/9)",
-                   "slots": args.slots, "hbm_budget_bytes": args.hbm_budget,
-                   "updates": sat_batched["updates"]},
-        "tuning": tuning,
-        "saturated": {"batched": sat_batched, "plain": sat_plain,
-                      "speedup_vs_plain": value / sat_plain["updates_per_s"]},
-        "rate_sweep": sweep,
+        "config": {"workload": leg["workload"], "slots": leg["slots"],
+                   "hbm_budget_bytes": leg["hbm_budget_bytes"], "updates": leg["updates"]},
+        "tuning": leg["tuning"],
+        "saturated": sat,
+        "rate_sweep": leg.get("rate_sweep", []),
     }
     print(json.dumps(out))
     return 0
@@ -637,6 +718,14 @@ def main() -> int:
     ap.add_argument("--heavy-frontier", type=int, default=0,
                     help="serving: frontier size at which a graph leaves the shared launch "
                          "(0 = library default, -1 = never)")
+    ap.add_argument("--no-batched", action="store_true",
+                    help="c2: skip the batched-workload comparison leg")
+    ap.add_argument("--no-serving", action="store_true",
+                    help="c2: skip the saturated serving comparison leg")
+    ap.add_argument("--batched-steps", type=int, default=1,
+                    help="c2: timed steps of the batched leg (each is all 256 graphs' streams)")
+    ap.add_argument("--batched-warmup", type=int, default=1,
+                    help="c2: untimed steps of the batched leg (the first captures)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="after the run, copy this many MiB device to device with the "
                          "library's copy kernel (reference point for HBM counters)")
@@ -659,6 +748,9 @@ def main() -> int:
     if args.workload == "stream":
         return stream_main(args)
 
+    t_start = time.perf_counter()
+    print(f"[bench] C2 plain loop: {args.nodes} vertices, {args.steps} steps", file=sys.stderr,
+          flush=True)
     inputs = L.generate(args.nodes, args.degree, args.batches, 42)
     cfg = L.make_config(inputs)
     ctx = L.Context(inputs, cfg)
@@ -736,8 +828,23 @@ def main() -> int:
         "random_access": ra_out.get("random_access"),
         "tuning": tuning,
     }
+    print(f"[bench] C2 done in {time.perf_counter() - t_start:.1f} s: {value / 1e6:.1f} M "
+          "updates/s", file=sys.stderr, flush=True)
     if not args.no_cpu:
         out.update(cpu_baselines(inputs))
+        print(f"[bench] CPU baselines done at {time.perf_counter() - t_start:.1f} s",
+              file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+
+    def progress(msg):
+        print(f"[bench {time.perf_counter() - t0:7.1f} s] {msg}", file=sys.stderr, flush=True)
+
+    # the north-star comparisons beside the plain C2 figures: the batched and
+    # the served forms against the plain one-graph loop on their workloads
+    if not args.no_batched:
+        out["batched"] = batched_leg(args, progress, autotune=False)
+    if not args.no_serving:
+        out["serving"] = serving_leg(args, progress, [])
     if args.calibrate_copy_mib > 0:
         out["copy_calibration"] = copy_calibration(args.calibrate_copy_mib)
     print(json.dumps(out))

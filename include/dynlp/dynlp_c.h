@@ -253,6 +253,12 @@ typedef struct dlp_tuning {
                            thread per core vertex, frontier as flags;
                            for long-lived frontiers such as limit
                            cycles): 1024 by default, -1 never            */
+  int waveRowMin;       /* multi-block kernel: rows longer than this (at
+                           most 64, the default) are summed by a whole
+                           wave instead of the vertex's lane group       */
+  int blockRowMin;      /* multi-block kernel: rows longer than this
+                           (>= 64; default 1024) are summed by a whole
+                           work-group; -1 never                          */
 } dlp_tuning;
 
 /* Uploads the full graph, true labels and every batch, allocates all device
@@ -569,6 +575,55 @@ int dlp_irlp_baseline(const dlp_config *cfg, const dlp_graph_view *fullGraph,
                       const float *trueLabels, const dlp_batch_view *batches,
                       const dlp_irlp_tuning *tuning, float *irlpLabels,
                       dlp_irlp_result *out);
+
+/* ====================================================================== */
+/* 8. Autotuner (no counterpart in the reference, which launches a        */
+/*    256-thread block per frontier vertex whatever the degrees,          */
+/*    iterativePropagation.cu:292-304)                                    */
+/* ====================================================================== */
+
+/* The shape class a tuning is remembered under. */
+typedef struct dlp_shape_key {
+  int logNodes;     /* ceil(log2 numNodes)                              */
+  int meanDegree;   /* stored entries / numNodes, rounded               */
+  int logMaxDegree; /* ceil(log2 (longest row + 1))                     */
+} dlp_shape_key;
+
+/* The degree-histogram part (host arithmetic, no GPU): the graph's shape
+ * key and the tuning a context of it starts from -- lanes per frontier
+ * vertex of the multi-block kernel (a lane group covers the
+ * 90th-percentile row in two passes, 4 to 32 lanes), the wave and
+ * work-group row tiers (the work-group tier only when a row is longer than
+ * 1,024 entries), the grid cap from the vertex count.  The LDS kernel's
+ * fields that need the device are left 0. */
+int dlp_autotune_heuristic(const dlp_graph_view *graph, dlp_shape_key *key, dlp_tuning *out);
+
+/* Times candidate launch shapes (lanes per vertex half, equal and double
+ * the current ones x LDS work-group shapes 1 and 3) on the context's first
+ * trialBatches batches (all when <= 0), keeps the fastest, stores it in the
+ * process's tuning table under the context's shape key and leaves the
+ * context reset.  Every candidate gives the same results. */
+int dlp_context_autotune(dlp_context *ctx, int trialBatches, dlp_tuning *chosen, float *bestMs);
+/* The context's shape key and where its tuning came from: 0 the
+ * heuristic, 1 the tuning table, 2 dlp_context_autotune. */
+int dlp_context_shape_key(const dlp_context *ctx, dlp_shape_key *key, int *source);
+
+/* Times candidate work-group shapes (3, 1) and lockstep hand-back
+ * frontiers (never, 65,536, 16,384) of the batched runner on its first
+ * trialBatches batch indices (all when <= 0), keeps the fastest, stores it
+ * in the tuning table under (log2 graph count, log2 total vertices) and
+ * leaves the runner reset. */
+int dlp_multi_autotune(dlp_multi *m, int trialBatches, dlp_multi_tuning *chosen, float *bestMs);
+/* Where the runner's tuning came from: 0 defaults, 1 table, 2 autotune. */
+int dlp_multi_tuning_source(const dlp_multi *m, int *source);
+
+/* The process's tuning table.  Contexts and runners created while it holds
+ * an entry for their shape start with that entry's tuning.  The library
+ * loads <its directory>/tuning/<gfx arch>.tune (if present) when the first
+ * context or runner is created.  Text, one entry per line. */
+int dlp_tuning_table_load(const char *path, int *entries);
+int dlp_tuning_table_save(const char *path, int *entries);
+int dlp_tuning_table_clear(void);
 
 #ifdef __cplusplus
 }

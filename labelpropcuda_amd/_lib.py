@@ -101,6 +101,8 @@ class Tuning(C.Structure):
         ("localLdsBytes", C.c_int),
         ("localShape", C.c_int),
         ("localOwnerAfter", C.c_int),
+        ("waveRowMin", C.c_int),
+        ("blockRowMin", C.c_int),
     ]
 
 
@@ -177,6 +179,14 @@ class IrlpResult(C.Structure):
         ("timeMs", C.c_float),
         ("loopMs", C.c_float),
         ("accuracy", C.c_float),
+    ]
+
+
+class ShapeKey(C.Structure):
+    _fields_ = [
+        ("logNodes", C.c_int),
+        ("meanDegree", C.c_int),
+        ("logMaxDegree", C.c_int),
     ]
 
 
@@ -271,6 +281,15 @@ SIGNATURES = {
     "dlp_device_free": (C.c_int, [_VP]),
     "dlp_copy_to_device": (C.c_int, [_VP, _VP, C.c_size_t]),
     "dlp_copy_to_host": (C.c_int, [_VP, _VP, C.c_size_t]),
+    "dlp_autotune_heuristic": (C.c_int, [C.POINTER(GraphView), C.POINTER(ShapeKey),
+                                         C.POINTER(Tuning)]),
+    "dlp_context_autotune": (C.c_int, [_VP, C.c_int, C.POINTER(Tuning), f32p]),
+    "dlp_context_shape_key": (C.c_int, [_VP, C.POINTER(ShapeKey), i32p]),
+    "dlp_multi_autotune": (C.c_int, [_VP, C.c_int, C.POINTER(MultiTuning), f32p]),
+    "dlp_multi_tuning_source": (C.c_int, [_VP, i32p]),
+    "dlp_tuning_table_load": (C.c_int, [C.c_char_p, i32p]),
+    "dlp_tuning_table_save": (C.c_int, [C.c_char_p, i32p]),
+    "dlp_tuning_table_clear": (C.c_int, []),
     "dlp_counter_guard_plan": (C.c_int, [C.c_int64, C.c_int64, C.c_int, i32p, i32p, i64p, i64p]),
     "dlp_context_set_counter_bases": (C.c_int, [_VP, C.c_int, C.c_int]),
     "dlp_context_counter_bases": (C.c_int, [_VP, i32p, i32p]),

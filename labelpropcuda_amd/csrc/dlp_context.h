@@ -36,7 +36,8 @@ struct dlp_context {
    * updates/s: fewer host round trips, and after a working set outgrew LDS
    * the LDS kernel is retried only after a chunk); local threshold and LDS
    * set at creation */
-  dlp_tuning tuning{8, 0, 1024, 0, 0, 1, kOwnerAfterDefault};
+  dlp_tuning tuning{8, 0, 1024, 0, 0, 1, kOwnerAfterDefault, kDefaultWaveRowMin,
+                    kDefaultBlockRowMin};
   LocalArgs local{};
   int localMaxBytes = 0;
   int nextBatch = 0;
@@ -48,6 +49,10 @@ struct dlp_context {
   std::vector<DevBatchRecord> records;
   std::vector<char> ran;
   dlp_trace *trace = nullptr;
+  /* shape class and where the tuning came from (0 heuristic, 1 table, 2
+   * autotune) */
+  dlp_shape_key shape{};
+  int tuningSource = 0;
   /* profiling: event-timed propagation launches */
   int profiling = 0;
   PropTiming timing;

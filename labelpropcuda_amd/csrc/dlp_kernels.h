@@ -34,7 +34,14 @@ struct PropArgs {
   int iterSizesCap;
   int profile;        /* accumulate the prof* work counters                    */
   float delta;
+  /* row tiers of the multi-block kernel: rows longer than waveRowMin (at
+   * most 64) are summed by a whole wave instead of a lane group, rows longer
+   * than blockRowMin (0 = never) by a whole work-group */
+  int waveRowMin;
+  int blockRowMin;
 };
+constexpr int kDefaultWaveRowMin = 64;
+constexpr int kDefaultBlockRowMin = 1024;
 
 /* one propagation iteration k (reads qcount[k%3]); groupLanes in
  * {4,8,16,32,64}; blocks of kPropagateThreads threads */

@@ -32,6 +32,7 @@
 #include <vector>
 
 #include "dlp_context.h"
+#include "dlp_tune.h"
 
 struct dlp_multi {
   int G = 0, maxB = 0;
@@ -56,6 +57,8 @@ struct dlp_multi {
   std::vector<hipEvent_t> evStart, evEnd;
   std::vector<char> ran;
   int nextBatch = 0;
+  int tuningSource = 0; /* 0 defaults, 1 tuning table, 2 dlp_multi_autotune */
+  int logGraphs = 0, logNodes = 0;
   /* run_next: descriptors of the graphs of one step */
   std::vector<GraphProp> props;     /* per graph, batch-independent      */
   GraphProp *dStep = nullptr;       /* device, G entries                 */
@@ -487,6 +490,16 @@ extern "C" int dlp_multi_create(dlp_multi **out, int numGraphs, const dlp_config
   }
 #undef MTRY
   host_reset(m);
+  {
+    long long total = 0;
+    for (dlp_context *c : m->ctx) total += c->N;
+    dlp_multi_shape(m->G, total, &m->logGraphs, &m->logNodes);
+    dlp_tuning_table_autoload();
+    dlp_multi_tuning t{};
+    if (dlp_tuning_lookup_multi(m->logGraphs, m->logNodes, &t) &&
+        dlp_multi_set_tuning(m, &t) == DLP_OK)
+      m->tuningSource = 1;
+  }
   *out = m;
   return DLP_OK;
 }
@@ -654,6 +667,79 @@ extern "C" int dlp_multi_counter_bases(dlp_multi *m, int g, int *stampBase, int 
   }
   DLP_HIP(hipStreamSynchronize(m->stream));
   return dlp_ctx_counter_bases(m->ctx[g], stampBase, localGen);
+}
+
+extern "C" int dlp_multi_tuning_source(const dlp_multi *m, int *source) {
+  if (!m || !source) {
+    dlp_set_error_("dlp_multi_tuning_source: NULL argument");
+    return DLP_ERR_INVALID;
+  }
+  *source = m->tuningSource;
+  return DLP_OK;
+}
+
+extern "C" int dlp_multi_autotune(dlp_multi *m, int trialBatches, dlp_multi_tuning *chosen,
+                                  float *bestMs) {
+  if (!m) {
+    dlp_set_error_("dlp_multi_autotune: NULL handle");
+    return DLP_ERR_STATE;
+  }
+  const int nb = trialBatches <= 0 ? m->maxB : std::min(trialBatches, m->maxB);
+  dlp_multi_tuning base{};
+  int rc = dlp_multi_get_tuning(m, &base);
+  if (rc != DLP_OK) return rc;
+  /* heavyFrontier 0 keeps the modes' defaults (lockstep: never) */
+  std::vector<dlp_multi_tuning> cand;
+  for (int shape : {3, 1})
+    for (int heavy : {0, 65536, 16384}) {
+      dlp_multi_tuning t = base;
+      t.localShape = shape;
+      t.heavyFrontier = heavy;
+      t.useGraphs = 1;
+      cand.push_back(t);
+    }
+  hipEvent_t e0, e1;
+  DLP_HIP(hipEventCreate(&e0));
+  DLP_HIP(hipEventCreate(&e1));
+  int best = -1;
+  float bestT = 0.0f;
+  auto pass = [&](float *ms) -> int {
+    int r = dlp_multi_reset(m);
+    if (r != DLP_OK) return r;
+    DLP_HIP(hipEventRecord(e0, m->stream));
+    for (int b = 0; b < nb; ++b)
+      if ((r = dlp_multi_run_batch(m, b)) != DLP_OK) return r;
+    DLP_HIP(hipEventRecord(e1, m->stream));
+    DLP_HIP(hipEventSynchronize(e1));
+    DLP_HIP(hipEventElapsedTime(ms, e0, e1));
+    return dlp_multi_synchronize(m);
+  };
+  for (size_t i = 0; i < cand.size() && rc == DLP_OK; ++i) {
+    /* set_tuning resets the heavy field only when non-zero: apply the
+     * default explicitly through the stored value */
+    m->heavy = cand[i].heavyFrontier;
+    if ((rc = dlp_multi_set_tuning(m, &cand[i])) != DLP_OK) break;
+    float ms = 0.0f;
+    if ((rc = pass(&ms)) != DLP_OK) break;  // captures the batch graphs
+    if ((rc = pass(&ms)) != DLP_OK) break;
+    if (best < 0 || ms < bestT) {
+      best = (int)i;
+      bestT = ms;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc != DLP_OK) return rc;
+  m->heavy = cand[best].heavyFrontier;
+  if ((rc = dlp_multi_set_tuning(m, &cand[best])) != DLP_OK) return rc;
+  if ((rc = dlp_multi_reset(m)) != DLP_OK) return rc;
+  dlp_multi_tuning got{};
+  if ((rc = dlp_multi_get_tuning(m, &got)) != DLP_OK) return rc;
+  dlp_tuning_put_multi(m->logGraphs, m->logNodes, got, bestT);
+  m->tuningSource = 2;
+  if (chosen) *chosen = got;
+  if (bestMs) *bestMs = bestT;
+  return DLP_OK;
 }
 
 extern "C" int dlp_multi_run_all(dlp_multi *m) {

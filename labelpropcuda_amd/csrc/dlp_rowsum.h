@@ -110,6 +110,41 @@ __device__ __forceinline__ void group_row_sums(int gl, int len, const Pos &pos, 
   group_tree<G>(ws, ss, W, S);
 }
 
+/* W and S of one row of any length by a whole work-group (at least 256
+ * threads): thread t < 256 accumulates slot t, then the reference's tree --
+ * strides 128 and 64 through LDS, 32 .. 1 by shuffles in the first wave.
+ * The result is valid in thread 0.  Every thread of the work-group must
+ * call it (it synchronises the work-group); ldsW / ldsS hold 256 floats. */
+template <class Pos>
+__device__ __forceinline__ void block_row_sums(int len, const Pos &pos, float *ldsW,
+                                               float *ldsS, float &W, float &S) {
+  const int t = threadIdx.x;
+  if (t < 256) {
+    float w, s;
+    slot_accumulate(pos, len, t, w, s);
+    ldsW[t] = w;
+    ldsS[t] = s;
+  }
+  __syncthreads();
+  if (t < 128) {
+    ldsW[t] = ldsW[t] + ldsW[t + 128];
+    ldsS[t] = ldsS[t] + ldsS[t + 128];
+  }
+  __syncthreads();
+  W = 0.0f;
+  S = 0.0f;
+  if (t < 64) {  // the whole first wave
+    W = ldsW[t] + ldsW[t + 64];
+    S = ldsS[t] + ldsS[t + 64];
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+      W = W + __shfl_down(W, off, 64);
+      S = S + __shfl_down(S, off, 64);
+    }
+  }
+  __syncthreads();  // ldsW / ldsS may be reused
+}
+
 constexpr int bit_reverse(int x, int bits) {
   int r = 0;
   for (int b = 0; b < bits; ++b) r |= ((x >> b) & 1) << (bits - 1 - b);

@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "dlp_kernels.h"
+#include "dlp_tune.h"
 
 /* ------------------------------------------------------------------------ */
 /* errors                                                                   */
@@ -448,6 +449,21 @@ extern "C" int dlp_context_create(dlp_context **out, const dlp_config *cfg,
     c->tuning.localLdsBytes = c->localMaxBytes = ldsBytes;
   }
 #undef TRY
+  /* launch shapes: the degree-histogram heuristic, then the tuning table's
+   * entry for this graph shape if there is one */
+  {
+    dlp_tuning h{};
+    if ((rc = dlp_autotune_heuristic(g, &c->shape, &h)) != DLP_OK ||
+        (rc = dlp_context_set_tuning(c, &h)) != DLP_OK) {
+      c->release();
+      delete c;
+      return rc;
+    }
+    dlp_tuning_table_autoload();
+    dlp_tuning t{};
+    if (dlp_tuning_lookup_ctx(c->shape, &t) && dlp_context_set_tuning(c, &t) == DLP_OK)
+      c->tuningSource = 1;
+  }
   if (hipHostMalloc((void **)&c->pinCtr, sizeof(DevCounters), hipHostMallocDefault) !=
       hipSuccess) {
     c->release();
@@ -529,6 +545,20 @@ extern "C" int dlp_context_set_tuning(dlp_context *c, const dlp_tuning *t) {
     n.localShape = t->localShape;
   }
   if (t->localOwnerAfter) n.localOwnerAfter = t->localOwnerAfter < 0 ? -1 : t->localOwnerAfter;
+  if (t->waveRowMin) {
+    if (t->waveRowMin < 1 || t->waveRowMin > 64) {
+      dlp_set_error_("waveRowMin must be in [1, 64]");
+      return DLP_ERR_INVALID;
+    }
+    n.waveRowMin = t->waveRowMin;
+  }
+  if (t->blockRowMin) {
+    if (t->blockRowMin != -1 && t->blockRowMin < 64) {
+      dlp_set_error_("blockRowMin must be -1 or at least 64");
+      return DLP_ERR_INVALID;
+    }
+    n.blockRowMin = t->blockRowMin;
+  }
   c->tuning = n;
   c->local.ldsWords = n.localLdsBytes / 4;
   c->local.shape = n.localShape - 1;
@@ -741,6 +771,8 @@ PropArgs dlp_ctx_prop_args(const dlp_context *c) {
   a.iterSizesCap = c->iterSizesCap;
   a.profile = c->profiling >= 2;
   a.delta = c->cfg.delta;
+  a.waveRowMin = c->tuning.waveRowMin;
+  a.blockRowMin = c->tuning.blockRowMin > 0 ? c->tuning.blockRowMin : 0;
   return a;
 }
 

@@ -325,6 +325,8 @@ int propagate_on(const PropScratch &ps, const int *rowPtr, const int *colInd, co
   a.iterSizesCap = 0;
   a.profile = 0;
   a.delta = delta;
+  a.waveRowMin = kDefaultWaveRowMin;
+  a.blockRowMin = kDefaultBlockRowMin;
 
   LocalArgs l{};
   l.ldsWords = ps.ldsBytes / 4;

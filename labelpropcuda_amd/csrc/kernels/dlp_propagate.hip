@@ -101,12 +101,20 @@ __device__ __forceinline__ int claim_stamp(int *stamp, int stampNext, bool filte
 /* frontier size from which k_propagate filters its claims */
 constexpr int kClaimFilterMin = 16384;
 
+/* hub rows (longer than PropArgs::blockRowMin) one work-group of the
+ * multi-block kernel sums together after its lane-group and wave work; more
+ * than this many in one work-group go to the wave path */
+constexpr int kHubCap = 64;
+
 template <int G>
 __global__ __launch_bounds__(kPT) void k_propagate(PropArgs a, int k) {
   constexpr int GPW = 64 / G; /* frontier vertices per wave */
   __shared__ int stageQ[kStageQ];
   __shared__ int stageT[kStageT];
   __shared__ int stageCount[2], stageBase[2];
+  __shared__ int hubList[kHubCap];
+  __shared__ int hubCount, hubChanged;
+  __shared__ float hubW[256], hubS[256];
   DevCounters *c = a.ctr;
   const int *in = (k & 1) ? a.q1 : a.q0;
   const int lane = threadIdx.x & 63;
@@ -131,7 +139,7 @@ __global__ __launch_bounds__(kPT) void k_propagate(PropArgs a, int k) {
     }
   }
   if (n == 0) return;  // block-uniform
-  if (threadIdx.x == 0) stageCount[0] = stageCount[1] = 0;
+  if (threadIdx.x == 0) stageCount[0] = stageCount[1] = hubCount = 0;
   __syncthreads();
 
   const int stampBase = c->stampBase;
@@ -144,7 +152,7 @@ __global__ __launch_bounds__(kPT) void k_propagate(PropArgs a, int k) {
 
   for (long long base = gwave * GPW; base < n; base += nwaves * GPW) {  // wave-uniform
     const long long item = base + grp;
-    const bool act = item < n;
+    bool act = item < n;
     int u = 0, beg = 0, len = 0;
     float oldLabel = 0.0f;
     if (act) {
@@ -159,11 +167,26 @@ __global__ __launch_bounds__(kPT) void k_propagate(PropArgs a, int k) {
         len = a.rowLen ? a.rowLen[u] : a.rowEnd[u] - beg;
       }
     }
-    /* rows of up to 64 entries by the lane group; longer ones (rare at the
-     * degrees of these graphs) by the whole wave, one after another, so the
-     * group path needs no registers for the 256-slot form and the kernel
-     * keeps a high occupancy */
-    const bool longRow = act && len > 64;
+    /* hub rows (longer than blockRowMin) wait for the whole work-group
+     * after this loop; a full hub list leaves them to the wave path */
+    bool deferred = false;
+    if (act && gl == 0 && a.blockRowMin > 0 && len > a.blockRowMin) {
+      const int slot = atomicAdd(&hubCount, 1);
+      if (slot < kHubCap) {
+        hubList[slot] = u;
+        deferred = true;
+      }
+    }
+    deferred = __shfl(deferred ? 1 : 0, grp * G, 64) != 0;
+    if (deferred) {
+      act = false;
+      len = 0;
+    }
+    /* rows of up to waveRowMin (<= 64) entries by the lane group; longer
+     * ones (rare at the degrees of these graphs) by the whole wave, one
+     * after another, so the group path needs no registers for the 256-slot
+     * form and the kernel keeps a high occupancy */
+    const bool longRow = act && len > a.waveRowMin;
     /* the lane's first column, read by the sum below and kept for the marks */
     const int col0 = (act && !longRow && gl < len) ? a.colInd[beg + gl] : -1;
     float W, S;
@@ -218,6 +241,56 @@ __global__ __launch_bounds__(kPT) void k_propagate(PropArgs a, int k) {
     }
   }
   __syncthreads();
+  /* hub rows: the whole work-group sums each (the reference's 256-slot
+   * order) and claims its neighbours */
+  const int nHub = min(hubCount, kHubCap);
+  for (int h = 0; h < nHub; ++h) {  // block-uniform
+    const int u = hubList[h];
+    int beg, len;
+    if (a.rowInfo) {
+      const int2 r = a.rowInfo[u];
+      beg = r.x;
+      len = r.y;
+    } else {
+      beg = a.rowBeg[u];
+      len = a.rowLen ? a.rowLen[u] : a.rowEnd[u] - beg;
+    }
+    float W, S;
+    dlp::block_row_sums(len, GlobalPos{a.colInd, a.values, Fold, a.status, beg}, hubW, hubS, W,
+                        S);
+    if (threadIdx.x == 0) {
+      const float oldLabel = Fold[u];
+      const float nl = dlp::new_label(W, S, oldLabel);
+      Fnew[u] = nl;
+      hubChanged = fabsf(nl - oldLabel) > a.delta;
+      profE += (unsigned)len;
+      if (hubChanged) profC += (unsigned)len + 1u;
+    }
+    __syncthreads();
+    if (hubChanged) {  // block-uniform
+      for (int p0 = 0; p0 <= len; p0 += kPT) {  // block-uniform
+        const int p = p0 + (int)threadIdx.x;
+        int v = -1;
+        bool pred = false;
+        if (p < len) {
+          v = a.colInd[beg + p];
+          pred = v >= 0 && !(a.status && a.status[v] == DLP_VERTEX_DELETED);
+        } else if (p == len) {
+          v = u;
+          pred = true;
+        }
+        const int old = pred ? claim_stamp(&a.iterStamp[v], stampNext, n >= kClaimFilterMin)
+                             : DLP_INT_MAX;
+        const bool app = pred && old < stampNext;
+        profA += app ? 1u : 0u;
+        stage_append<kStageQ>(stageQ, &stageCount[0], outCount, out, app, v);
+        if (a.touched)
+          stage_append<kStageT>(stageT, &stageCount[1], &c->touched, a.touched,
+                                app && old < stampBase, v);
+      }
+    }
+    __syncthreads();  // hubChanged is read by every thread before the next hub
+  }
   stage_flush<kStageQ>(stageQ, stageCount[0], outCount, out, &stageBase[0]);
   if (a.touched) stage_flush<kStageT>(stageT, stageCount[1], &c->touched, a.touched, &stageBase[1]);
   if (a.profile) {  // LDS sums per block, then one global atomic per counter per block

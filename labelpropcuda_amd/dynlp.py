@@ -375,12 +375,14 @@ class Context:
         return b.value
 
     def set_tuning(self, groupLanes=0, blocks=0, chunk=0, localThreshold=0, localLdsBytes=0,
-                   localShape=0, localOwnerAfter=0):
+                   localShape=0, localOwnerAfter=0, waveRowMin=0, blockRowMin=0):
         """0 keeps a setting; localThreshold < 0 disables the LDS-resident
         propagation kernel; localShape is 1-based; localOwnerAfter -1 runs
-        the LDS-resident kernel with queue iterations only."""
+        the LDS-resident kernel with queue iterations only; rows longer than
+        waveRowMin / blockRowMin go to a wave / a work-group (blockRowMin -1:
+        never)."""
         t = _lib.Tuning(groupLanes, blocks, chunk, localThreshold, localLdsBytes, localShape,
-                        localOwnerAfter)
+                        localOwnerAfter, waveRowMin, blockRowMin)
         check(self._lib.dlp_context_set_tuning(self._h, C.byref(t)))
 
     def tuning(self) -> dict:
@@ -429,6 +431,23 @@ class Context:
         s = _lib.BatchStats()
         check(self._lib.dlp_context_batch_stats(self._h, b, C.byref(s)))
         return {name: getattr(s, name) for name, _ in s._fields_}
+
+    def autotune(self, trialBatches: int = 0):
+        """Times candidate launch shapes on the first trialBatches batches
+        (all when 0), keeps the fastest and stores it in the process's tuning
+        table; returns (tuning, ms of the best trial)."""
+        t = _lib.Tuning()
+        ms = C.c_float()
+        check(self._lib.dlp_context_autotune(self._h, trialBatches, C.byref(t), C.byref(ms)))
+        return {name: getattr(t, name) for name, _ in t._fields_}, ms.value
+
+    def shape_key(self):
+        """(shape key, tuning source: 'heuristic' | 'table' | 'autotune')."""
+        k = _lib.ShapeKey()
+        src = C.c_int()
+        check(self._lib.dlp_context_shape_key(self._h, C.byref(k), C.byref(src)))
+        return ({name: getattr(k, name) for name, _ in k._fields_},
+                ("heuristic", "table", "autotune")[src.value])
 
     def set_counter_bases(self, stampBase: int, localGen: int):
         """Moves the iteration-stamp base and LDS generation counter up
@@ -520,6 +539,19 @@ class Multi:
         check(self._lib.dlp_multi_device_bytes(self._h, C.byref(v)))
         return v.value
 
+    def autotune(self, trialBatches: int = 0):
+        """Times work-group shapes x hand-back frontiers on the first
+        trialBatches batch indices (all when 0); keeps the fastest."""
+        t = _lib.MultiTuning()
+        ms = C.c_float()
+        check(self._lib.dlp_multi_autotune(self._h, trialBatches, C.byref(t), C.byref(ms)))
+        return {name: getattr(t, name) for name, _ in t._fields_}, ms.value
+
+    def tuning_source(self) -> str:
+        src = C.c_int()
+        check(self._lib.dlp_multi_tuning_source(self._h, C.byref(src)))
+        return ("defaults", "table", "autotune")[src.value]
+
     def set_counter_bases(self, g: int, stampBase: int, localGen: int):
         check(self._lib.dlp_multi_set_counter_bases(self._h, g, stampBase, localGen))
 
@@ -546,6 +578,31 @@ class Multi:
         check(self._lib.dlp_multi_serve(self._h, reqs, n, C.byref(cfg),
                                         _ptr(done, C.c_double), C.byref(steps)))
         return done[:n], steps.value
+
+
+def autotune_heuristic(inputs: Inputs):
+    """The degree-histogram heuristic for a graph: (shape key, tuning)."""
+    k, t = _lib.ShapeKey(), _lib.Tuning()
+    g = inputs.graph_view()
+    check(load().dlp_autotune_heuristic(C.byref(g), C.byref(k), C.byref(t)))
+    return ({name: getattr(k, name) for name, _ in k._fields_},
+            {name: getattr(t, name) for name, _ in t._fields_})
+
+
+def tuning_table_load(path: str) -> int:
+    n = C.c_int()
+    check(load().dlp_tuning_table_load(path.encode(), C.byref(n)))
+    return n.value
+
+
+def tuning_table_save(path: str) -> int:
+    n = C.c_int()
+    check(load().dlp_tuning_table_save(path.encode(), C.byref(n)))
+    return n.value
+
+
+def tuning_table_clear() -> None:
+    check(load().dlp_tuning_table_clear())
 
 
 def counter_guard_plan(stampBound: int, genBound: int, maxIterations: int) -> dict:

@@ -195,10 +195,10 @@ def test_owner_mode_and_queue_iterations(gpu_available, c1_inputs, shape, owner,
     assert prof["localOwnerUpdates"] <= prof["localUpdates"]
 
 
-def hub_inputs(tmp_path):
-    """3,000 vertices at degree ~6 plus two hubs (vertex 0 adjacent to 1..700,
-    vertex 1 to 2..320), read back through the graph text files, with the
-    reference batch generator over it: once most vertices are active the
+def hub_inputs(tmp_path, hubs=((0, 700), (1, 320))):
+    """3,000 vertices at degree ~6 plus hubs (by default vertex 0 adjacent to
+    1..700, vertex 1 to 2..320), read back through the graph text files, with
+    the reference batch generator over it: once most vertices are active the
     hubs' rows exceed 256 entries."""
     rng = np.random.default_rng(3)
     n = 3000
@@ -206,7 +206,8 @@ def hub_inputs(tmp_path):
     for u, v in rng.integers(0, n, (9000, 2)).tolist():
         if u != v:
             pairs.add((min(u, v), max(u, v)))
-    pairs |= {(0, v) for v in range(1, 701)} | {(1, v) for v in range(2, 321)}
+    for h, k in hubs:
+        pairs |= {(h, v) for v in range(h + 1, h + 1 + k)}
     rows = [[] for _ in range(n)]
     for u, v in sorted(pairs):
         w = float(rng.uniform(0.01, 1.0))
@@ -239,6 +240,21 @@ def test_rows_beyond_64_and_256_entries(gpu_available, tmp_path, lanes, threshol
     run_engine_mix(inp, 3000, groupLanes=lanes, localThreshold=threshold)
     dense = L.generate(2000, 80, 4, 5)  # most rows between 65 and 121 entries
     run_engine_mix(dense, 1000, groupLanes=lanes, localThreshold=threshold)
+
+
+@pytest.mark.parametrize("waveRowMin,blockRowMin,lanes", [
+    (64, 64, 8), (64, 300, 8), (64, 1024, 8), (16, 300, 4), (4, 128, 16), (64, -1, 8),
+])
+def test_row_tiers_lane_wave_block(gpu_available, tmp_path, waveRowMin, blockRowMin, lanes):
+    """Rows are summed by a lane group, by a whole wave (longer than
+    waveRowMin) or by a whole work-group (longer than blockRowMin): hubs of
+    2,500, 1,200 and 400 entries and the degree-6 rest give the oracle's
+    labels, iterations and frontiers under every split of the three tiers."""
+    inp = hub_inputs(tmp_path, hubs=((0, 2500), (3, 1200), (7, 400)))
+    deg = np.diff(np.asarray(inp.rowPtr))
+    assert deg[0] >= 2500 and deg[3] >= 1200 and deg[7] >= 400
+    run_engine_mix(inp, 3000, groupLanes=lanes, localThreshold=-1, waveRowMin=waveRowMin,
+                   blockRowMin=blockRowMin)
 
 
 def test_replay_is_deterministic(gpu_available, c1_inputs):
