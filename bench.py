@@ -685,6 +685,9 @@ def main() -> int:
     ap.add_argument("--local-owner-after", type=int, default=0,
                     help="iterations of an LDS-resident pass before owner mode (0 = default, "
                          "-1 = never)")
+    ap.add_argument("--mask-owner-max", type=int, default=0,
+                    help="owner mode runs in the register-mask LDS kernels on frontiers of at "
+                         "most this many vertices (0 = default, -1 = never)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--workload", choices=("c2", "batched", "serving", "irlp", "stream"),
                     default="c2",
@@ -755,7 +758,8 @@ def main() -> int:
     cfg = L.make_config(inputs)
     ctx = L.Context(inputs, cfg)
     ctx.set_tuning(args.lanes, args.blocks, args.chunk, args.local_threshold,
-                   localShape=args.local_shape, localOwnerAfter=args.local_owner_after)
+                   localShape=args.local_shape, localOwnerAfter=args.local_owner_after,
+                   maskOwnerMax=args.mask_owner_max)
     tuning = ctx.tuning()
 
     for _ in range(args.warmup):

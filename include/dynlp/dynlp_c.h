@@ -259,6 +259,12 @@ typedef struct dlp_tuning {
   int blockRowMin;      /* multi-block kernel: rows longer than this
                            (>= 64; default 1024) are summed by a whole
                            work-group; -1 never                          */
+  int maskOwnerMax;     /* owner mode on a frontier of at most this many
+                           vertices (<= 512, the default) runs in the
+                           register-mask LDS kernels (one core vertex per
+                           thread, rows in registers, frontier as 64-bit
+                           masks, one or no barrier per iteration); -1
+                           never                                         */
 } dlp_tuning;
 
 /* Uploads the full graph, true labels and every batch, allocates all device
@@ -335,6 +341,9 @@ typedef struct dlp_prop_profile {
   int64_t localAppended;
   int64_t localOwnerUpdates; /* of localUpdates: processed in owner mode   */
   int64_t localBuilds;       /* LDS working sets built (passes)           */
+  int64_t localWaveUpdates;  /* of localOwnerUpdates: processed by the
+                                register-mask owner kernels (cores of at
+                                most 512 vertices)                        */
 } dlp_prop_profile;
 
 /* Sets the profiling level and clears the accumulated profile:

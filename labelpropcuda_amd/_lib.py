@@ -103,6 +103,7 @@ class Tuning(C.Structure):
         ("localOwnerAfter", C.c_int),
         ("waveRowMin", C.c_int),
         ("blockRowMin", C.c_int),
+        ("maskOwnerMax", C.c_int),
     ]
 
 
@@ -159,6 +160,7 @@ class PropProfile(C.Structure):
         ("localAppended", C.c_int64),
         ("localOwnerUpdates", C.c_int64),
         ("localBuilds", C.c_int64),
+        ("localWaveUpdates", C.c_int64),
     ]
 
 

@@ -92,10 +92,11 @@ int parse_table(FILE *f) {
     dlp_multi_tuning mt{};
     float ms = 0.0f;
     int lg = 0, ln = 0;
-    if (std::sscanf(line, "context %d %d %d : %d %d %d %d %d %d %d %d %d ms=%f", &k.logNodes,
-                    &k.meanDegree, &k.logMaxDegree, &t.propGroupLanes, &t.propBlocks,
-                    &t.chunkIterations, &t.localThreshold, &t.localLdsBytes, &t.localShape,
-                    &t.localOwnerAfter, &t.waveRowMin, &t.blockRowMin, &ms) == 13) {
+    if (std::sscanf(line, "context %d %d %d : %d %d %d %d %d %d %d %d %d %d ms=%f",
+                    &k.logNodes, &k.meanDegree, &k.logMaxDegree, &t.propGroupLanes,
+                    &t.propBlocks, &t.chunkIterations, &t.localThreshold, &t.localLdsBytes,
+                    &t.localShape, &t.localOwnerAfter, &t.waveRowMin, &t.blockRowMin,
+                    &t.maskOwnerMax, &ms) == 14) {
       put_ctx(k, t, ms);
       ++n;
     } else if (std::sscanf(line, "multi %d %d : %d %d %d %d %d ms=%f", &lg, &ln, &mt.localShape,
@@ -222,6 +223,7 @@ extern "C" int dlp_autotune_heuristic(const dlp_graph_view *g, dlp_shape_key *ke
   t.waveRowMin = kDefaultWaveRowMin;
   /* the work-group tier only exists for graphs with hub rows */
   t.blockRowMin = maxDeg > kDefaultBlockRowMin ? kDefaultBlockRowMin : -1;
+  t.maskOwnerMax = kMaskOwnerMax;
   *out = t;
   return DLP_OK;
 }
@@ -350,14 +352,15 @@ extern "C" int dlp_tuning_table_save(const char *path, int *entries) {
                "# libdynlp tuning table: timed launch shapes per graph shape\n"
                "# context <log2 N> <mean degree> <log2 (max degree + 1)> : <propGroupLanes> "
                "<propBlocks> <chunkIterations> <localThreshold> <localLdsBytes> <localShape> "
-               "<localOwnerAfter> <waveRowMin> <blockRowMin> ms=<trial ms>\n"
+               "<localOwnerAfter> <waveRowMin> <blockRowMin> <maskOwnerMax> ms=<trial ms>\n"
                "# multi <log2 graphs> <log2 total vertices> : <localShape> <thresholdPercent> "
                "<backoff> <useGraphs> <heavyFrontier> ms=<trial ms>\n");
   for (const CtxEntry &e : g_ctx)
-    std::fprintf(f, "context %d %d %d : %d %d %d %d %d %d %d %d %d ms=%.3f\n", e.key.logNodes,
-                 e.key.meanDegree, e.key.logMaxDegree, e.t.propGroupLanes, e.t.propBlocks,
-                 e.t.chunkIterations, e.t.localThreshold, e.t.localLdsBytes, e.t.localShape,
-                 e.t.localOwnerAfter, e.t.waveRowMin, e.t.blockRowMin, e.ms);
+    std::fprintf(f, "context %d %d %d : %d %d %d %d %d %d %d %d %d %d ms=%.3f\n",
+                 e.key.logNodes, e.key.meanDegree, e.key.logMaxDegree, e.t.propGroupLanes,
+                 e.t.propBlocks, e.t.chunkIterations, e.t.localThreshold, e.t.localLdsBytes,
+                 e.t.localShape, e.t.localOwnerAfter, e.t.waveRowMin, e.t.blockRowMin,
+                 e.t.maskOwnerMax, e.ms);
   for (const MultiEntry &e : g_multi)
     std::fprintf(f, "multi %d %d : %d %d %d %d %d ms=%.3f\n", e.logGraphs, e.logNodes,
                  e.t.localShape, e.t.thresholdPercent, e.t.backoff, e.t.useGraphs,

@@ -37,7 +37,7 @@ struct dlp_context {
    * the LDS kernel is retried only after a chunk); local threshold and LDS
    * set at creation */
   dlp_tuning tuning{8, 0, 1024, 0, 0, 1, kOwnerAfterDefault, kDefaultWaveRowMin,
-                    kDefaultBlockRowMin};
+                    kDefaultBlockRowMin, kMaskOwnerMax};
   LocalArgs local{};
   int localMaxBytes = 0;
   int nextBatch = 0;

@@ -68,6 +68,8 @@ struct DevCounters {
   unsigned long long profLocalAppended;
   unsigned long long profLocalOwnerUpdates; /* of profLocalUpdates: owner mode */
   unsigned long long profLocalBuilds;       /* LDS working sets built         */
+  unsigned long long profLocalWaveUpdates;  /* of profLocalOwnerUpdates: in the
+                                               single-wave owner kernel      */
 };
 
 /* exit reasons of the LDS-resident propagation kernel */
@@ -75,8 +77,11 @@ enum {
   DLP_LOCAL_CONVERGED = 1, /* next frontier empty                            */
   DLP_LOCAL_CAPPED = 2,    /* maxIterations reached                          */
   DLP_LOCAL_TOO_BIG = 3,   /* working set does not fit; continue multi-block */
-  DLP_LOCAL_LONG = 4       /* queue kernel ran LocalArgs::ownerAfter iterations:
+  DLP_LOCAL_LONG = 4,      /* queue kernel ran LocalArgs::ownerAfter iterations:
                               continue with the owner kernel                 */
+  DLP_LOCAL_WIDE = 5       /* single-wave owner kernel: the core is too large
+                              (or has long rows or ineligible vertices) for
+                              it; continue with the work-group owner kernel  */
 };
 
 /* per-batch record kept on the device, read back once per run */

@@ -79,6 +79,14 @@ struct LocalArgs {
   int ownerAfter;        /* iterations of a launch of the queue kernel before the
                             host goes on with the owner kernel; < 0: never     */
 };
+/* LocalArgs::shape of the register-mask owner kernels (rows padded to 16
+ * records, one core vertex per thread): kLocalMaskShape + level, level 0 =
+ * one wave (cores of at most 64 vertices), 1 = 256 threads, 2 = 512;
+ * chosen by the host for owner mode on small frontiers, not tuning shapes */
+constexpr int kLocalMaskShape = 100;
+constexpr int kMaskLevels = 3;
+constexpr int kMaskLevelThreads[kMaskLevels] = {64, 256, 512};
+constexpr int kMaskOwnerMax = 512;
 /* default LocalArgs::ownerAfter: only long-lived frontiers (C2's limit
  * cycles run 100,000 iterations in one launch) go on in owner mode */
 constexpr int kOwnerAfterDefault = 1024;
@@ -186,6 +194,8 @@ struct PropLoopConfig {
                          kMinPropagateBlocks                                 */
   int maxChunk;       /* multi-block iterations between host checks          */
   int localThreshold; /* frontier size at or below which the LDS kernel runs */
+  int maskOwnerMax;   /* owner mode starts in a register-mask kernel on
+                         frontiers of at most this many vertices (<= 0 never) */
 };
 
 /* Runs propagation iterations 0 .. until the frontier empties or maxIter,

@@ -81,6 +81,8 @@ int dlp_run_propagation(const PropArgs &a, const LocalArgs *l, int maxIter,
   int chunk = 2;
   int localAfter = 0; /* no local launch before this iteration (after an overflow) */
   bool owner = false;  /* the last queue launch of the LDS kernel ran long */
+  int maskMin = 0;     /* smallest register-mask owner kernel to try (a kernel
+                          that found the core too wide moves it up) */
   const int maxChunk = std::max(1, cfg.maxChunk);
   if ((rc = read_ctr(a.ctr, pin, s)) != DLP_OK) return rc;
   while (k < maxIter) {
@@ -92,8 +94,21 @@ int dlp_run_propagation(const PropArgs &a, const LocalArgs *l, int maxIter,
       if ((long long)pin->localGen + dlp_gens_per_batch(maxIter) >= kGenLimit &&
           (rc = dlp_restart_local_gens(*l, a.ctr, s)) != DLP_OK)
         return rc;
+      /* owner mode on a small frontier: a register-mask kernel, sized for a
+       * core of about twice the frontier (a limit cycle alternates between
+       * two frontiers), at least maskMin */
+      LocalArgs la = *l;
+      int level = kMaskLevels;
+      if (owner && n <= std::min(cfg.maskOwnerMax, kMaskOwnerMax)) {
+        level = maskMin;
+        while (level < kMaskLevels && 2 * n > kMaskLevelThreads[level]) ++level;
+        /* the multi-wave mask kernels read the next frontier off symmetric
+         * rows: not for a caller's CSR (status array given) */
+        if (a.status && level > 0) level = kMaskLevels;
+        if (level < kMaskLevels) la.shape = kLocalMaskShape + level;
+      }
       if ((rc = begin_timed(timing, s, true, 1)) != DLP_OK) return rc;
-      if ((rc = dlp_launch_propagate_local(a, *l, k, maxIter, owner, s)) != DLP_OK) return rc;
+      if ((rc = dlp_launch_propagate_local(a, la, k, maxIter, owner, s)) != DLP_OK) return rc;
       if ((rc = end_timed(timing, s)) != DLP_OK) return rc;
       if ((rc = read_ctr(a.ctr, pin, s)) != DLP_OK) return rc;
       const int next = pin->nextIter;
@@ -106,7 +121,13 @@ int dlp_run_propagation(const PropArgs &a, const LocalArgs *l, int maxIter,
         chunk = 2;
         owner = false;
       }
-      if (pin->localReason == DLP_LOCAL_LONG) owner = true;
+      if (pin->localReason == DLP_LOCAL_LONG) {
+        owner = true;
+        maskMin = 0;
+      }
+      /* the core did not fit the mask kernel: the next larger one, or the
+       * flag-based owner kernel */
+      if (pin->localReason == DLP_LOCAL_WIDE) maskMin = level + 1;
       k = next;
       if (pin->localReason == DLP_LOCAL_CONVERGED || pin->localReason == DLP_LOCAL_CAPPED)
         break;

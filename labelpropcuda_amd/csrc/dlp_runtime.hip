@@ -207,6 +207,7 @@ int run_propagation(dlp_context *c, const PropArgs &a, int maxIter, int startK =
   cfg.blocks = c->tuning.propBlocks > 0 ? c->tuning.propBlocks : dlp_auto_prop_blocks(c->N);
   cfg.maxChunk = c->tuning.chunkIterations;
   cfg.localThreshold = c->tuning.localThreshold;
+  cfg.maskOwnerMax = c->tuning.maskOwnerMax;
   /* the local kernel encodes outside vertices as -2 - (2 * id + 1) */
   const bool local = c->tuning.localThreshold > 0 && c->N < (1 << 30);
   return dlp_run_propagation(a, local ? &c->local : nullptr, maxIter, cfg, c->pinCtr, c->stream,
@@ -552,6 +553,13 @@ extern "C" int dlp_context_set_tuning(dlp_context *c, const dlp_tuning *t) {
     }
     n.waveRowMin = t->waveRowMin;
   }
+  if (t->maskOwnerMax) {
+    if (t->maskOwnerMax != -1 && (t->maskOwnerMax < 1 || t->maskOwnerMax > kMaskOwnerMax)) {
+      dlp_set_errorf_("maskOwnerMax must be -1 or in [1, %d]", kMaskOwnerMax);
+      return DLP_ERR_INVALID;
+    }
+    n.maskOwnerMax = t->maskOwnerMax;
+  }
   if (t->blockRowMin) {
     if (t->blockRowMin != -1 && t->blockRowMin < 64) {
       dlp_set_error_("blockRowMin must be -1 or at least 64");
@@ -767,8 +775,10 @@ PropArgs dlp_ctx_prop_args(const dlp_context *c) {
   a.q1 = d.q1;
   a.touched = d.touched;
   a.ctr = d.ctr;
-  a.iterSizes = c->iterSizes;
-  a.iterSizesCap = c->iterSizesCap;
+  /* per-iteration frontier sizes only for a trace: a global store per
+   * iteration would also hold up the LDS kernels' work-group barriers */
+  a.iterSizes = c->trace ? c->iterSizes : nullptr;
+  a.iterSizesCap = c->trace ? c->iterSizesCap : 0;
   a.profile = c->profiling >= 2;
   a.delta = c->cfg.delta;
   a.waveRowMin = c->tuning.waveRowMin;
@@ -802,6 +812,7 @@ extern "C" int dlp_context_set_profiling(dlp_context *c, int enable) {
   ctr.profEdges = ctr.profChangedEdges = ctr.profAppended = 0;
   ctr.profLocalUpdates = ctr.profLocalEdges = ctr.profLocalChangedEdges = 0;
   ctr.profLocalAppended = ctr.profLocalOwnerUpdates = ctr.profLocalBuilds = 0;
+  ctr.profLocalWaveUpdates = 0;
   DLP_HIP(hipMemcpy(c->d.ctr, &ctr, sizeof(ctr), hipMemcpyHostToDevice));
   c->profiling = enable;
   c->timing.perChunk = enable == 1;
@@ -832,6 +843,7 @@ extern "C" int dlp_context_prop_profile(dlp_context *c, dlp_prop_profile *out) {
   out->localAppended = (int64_t)ctr.profLocalAppended;
   out->localOwnerUpdates = (int64_t)ctr.profLocalOwnerUpdates;
   out->localBuilds = (int64_t)ctr.profLocalBuilds;
+  out->localWaveUpdates = (int64_t)ctr.profLocalWaveUpdates;
   return DLP_OK;
 }
 

@@ -344,6 +344,7 @@ int propagate_on(const PropScratch &ps, const int *rowPtr, const int *colInd, co
   cfg.groupLanes = 8;
   cfg.blocks = dlp_auto_prop_blocks(numNodes);
   cfg.maxChunk = 1024;
+  cfg.maskOwnerMax = kMaskOwnerMax;
   {
     int nnz = 0;
     DLP_HIP(hipMemcpyAsync(&nnz, rowPtr + numNodes, sizeof(int), hipMemcpyDeviceToHost, s));

@@ -131,6 +131,9 @@ __global__ void k_reset(DState d) {
     z.profLocalEdges = d.ctr->profLocalEdges;
     z.profLocalChangedEdges = d.ctr->profLocalChangedEdges;
     z.profLocalAppended = d.ctr->profLocalAppended;
+    z.profLocalOwnerUpdates = d.ctr->profLocalOwnerUpdates;
+    z.profLocalBuilds = d.ctr->profLocalBuilds;
+    z.profLocalWaveUpdates = d.ctr->profLocalWaveUpdates;
     z.localGen = d.ctr->localGen;  // the core generations stored per vertex stay valid
     *d.ctr = z;
   }

@@ -402,7 +402,8 @@ __device__ int block_scan_lds(const int *a, int *out, int n, int *tmp /* >= 17 i
 
 enum {
   SH_GEN, SH_CC, SH_NCAND, SH_NCORE, SH_BIG, SH_REBUILD, SH_NOVF, SH_HASLONG,
-  SH_NHE, SH_NHI, SH_QC0, SH_QC1, SH_QC2, SH_NLONG0, SH_NLONG1, SH_INELIG, SH_COUNT
+  SH_NHE, SH_NHI, SH_QC0, SH_QC1, SH_QC2, SH_NLONG0, SH_NLONG1, SH_INELIG, SH_MAXLEN,
+  SH_COUNT
 };
 
 /* state of one local iteration shared by its row paths */
@@ -524,6 +525,9 @@ constexpr int kRowHasOut = 0x40000000;
 /* LDS words past the working set (rounding of the aligned sections) */
 constexpr int kLdsSlack = 64;
 
+/* most waves of a register-mask owner work-group (512 threads) */
+constexpr int kMaskWavesMax = 8;
+
 /* static LDS of the local passes */
 struct LocalShared {
   int sh[SH_COUNT];
@@ -533,11 +537,18 @@ struct LocalShared {
   int dummyOut[64];
   int ownerCount;     /* queue rebuilt from the owner-mode flags */
   unsigned long long ownerSum; /* frontier sizes of the owner-mode iterations */
+  /* register-mask owner mode with several waves: per wave and iteration
+   * parity, the changed-vertex mask, whether a mark left the core, and the
+   * size of the wave's next frontier */
+  unsigned long long xChanged[2][kMaskWavesMax];
+  unsigned char xOut[2][kMaskWavesMax];
+  unsigned char xCount[2][kMaskWavesMax]; /* 0 .. 64 */
 };
 
 /* lane-local work counters of the local passes */
 struct LocalProf {
   unsigned U = 0, E = 0, C = 0, A = 0, D = 0; /* D: updates run in owner mode */
+  unsigned Wv = 0;                            /* of D: in the single-wave owner mode */
 };
 
 /* One owned vertex per lane in owner mode (all 64 lanes call it; `act`
@@ -830,11 +841,448 @@ __device__ __forceinline__ int owner_iterations(
   return reason;
 }
 
+/* the OR of a 64-bit mask over the 64 lanes of the wave, in every lane */
+__device__ __forceinline__ unsigned long long wave_or64(unsigned long long m) {
+  unsigned lo = (unsigned)m, hi = (unsigned)(m >> 32);
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    lo |= (unsigned)__shfl_xor((int)lo, off, 64);
+    hi |= (unsigned)__shfl_xor((int)hi, off, 64);
+  }
+  return ((unsigned long long)hi << 32) | lo;
+}
+
+/* New label of a register-held row (its first LL <= L records used; rows
+ * are padded with null records): the neighbour labels come from the LDS
+ * label array Fo.  Lanes without a flagged vertex read the null slot (one
+ * address: a broadcast, no bank traffic) and their result is not used. */
+template <int LL, int L>
+__device__ __forceinline__ float mask_row_label(const int (&src)[L], const float (&w)[L],
+                                                const float *Fo, int nullSlot, bool f,
+                                                float old) {
+  float ff[LL];
+#pragma unroll
+  for (int p = 0; p < LL; ++p) ff[p] = Fo[f ? src[p] : nullSlot];
+  float W, Sm;
+  dlp::lane_row_sums<LL>(
+      LL,
+      [&](int p, float &ww, float &ss) {
+        ww = f ? w[p] : 0.0f;
+        ss = ww * ff[p];
+      },
+      W, Sm);
+  return dlp::new_label(W, Sm, old);
+}
+
+/* mask_row_label with the narrowest row width that covers the wave's
+ * flagged rows (wave-uniform choice); all lanes of the wave call it */
+template <int L>
+__device__ __forceinline__ float mask_row_label_any(const int (&src)[L], const float (&w)[L],
+                                                    const float *Fo, int nullSlot, bool f,
+                                                    int len, float old) {
+  static_assert(L == 8 || L == 16, "register rows of 8 or 16 records");
+  if (!__any(f && len > 4)) return mask_row_label<4, L>(src, w, Fo, nullSlot, f, old);
+  if (L == 8 || !__any(f && len > 8)) return mask_row_label<8, L>(src, w, Fo, nullSlot, f, old);
+  return mask_row_label<L, L>(src, w, Fo, nullSlot, f, old);
+}
+
+/* Single-wave owner mode: the work-group is one wave (64 threads) and the
+ * core at most 64 * V vertices, every core row at most L <= SR entries
+ * (stored padded in recS), every core vertex eligible.  Lane `lane` owns
+ * core vertices lane + 64 r (r < V) and keeps in registers their row
+ * records, both labels, and for each of them the core vertices its row
+ * names, as V 64-bit masks.  The frontier of an iteration is V 64-bit masks
+ * (uniform).  The next one is the changed vertices plus every core vertex
+ * some changed vertex's row names: on the pipeline's CSR, which is
+ * symmetric (a row names v iff v's row names it), that is a vertex whose
+ * own row mask meets the changed mask -- register operations only; on a
+ * caller's CSR (status array given) the changed vertices' row masks are
+ * OR-reduced over the wave.  No flag stores, no atomics, no barrier between
+ * waves: an iteration is a short chain of register and LDS operations.
+ * New labels go to the LDS label array, where neighbours read them, and to
+ * the registers.  Iterations, arithmetic and the state left behind (labels,
+ * the queue of iteration k and its count, stamps of the vertices that
+ * entered a frontier, the counters) are those of owner_iterations; marks of
+ * eligible vertices outside the core go through mark_outside and end the
+ * pass (rebuild).  Returns the DLP_LOCAL_* reason or 0 (rebuild). */
+template <int SR, int V, int L>
+__device__ __forceinline__ int wave_owner_iterations(
+    const PropArgs &a, const LocalArgs &l, LocalShared &S, int &k, int maxIter, int kBuild,
+    int nCore, int nHE, int nullSlot, int stampBase, const int *rlL, const int2 *recS,
+    int *stampL, int *q0L, int *q1L, float *lab0L, float *lab1L, const int *haloE,
+    unsigned long long &passUpdates, LocalProf &prof) {
+  static_assert(L <= SR, "rows are read from the padded short-row records");
+  constexpr int RS = SR + 1;
+  int *sh = S.sh;
+  DevCounters *c = a.ctr;
+  const int lane = threadIdx.x;  // the work-group is one wave
+  int *dumOut = S.dummyOut + lane;
+  const bool symmetric = a.status == nullptr;  // the pipeline's active CSR
+  /* frontier k: the queue's core indices as flags in the other queue
+   * array, then as masks */
+  const int nq = sh[SH_QC0 + k % 3];
+  {
+    const int *qin = (k & 1) ? q1L : q0L;
+    int *flags = (k & 1) ? q0L : q1L;
+    for (int i = lane; i < nCore; i += 64) flags[i] = 0;
+    __syncthreads();
+    for (int i = lane; i < nq; i += 64) flags[qin[i]] = 1;
+    __syncthreads();
+  }
+  const int *flags0 = (k & 1) ? q0L : q1L;
+  unsigned long long fm[V], ownMask[V];
+  int src[V][L];
+  float w[V][L];
+  unsigned long long out[V][V]; /* out[r][q]: core vertices q*64+j named by row r */
+  float l0[V], l1[V];
+  int len[V];
+  unsigned outPos[V]; /* row positions naming eligible vertices outside the core */
+  bool entered[V];
+#pragma unroll
+  for (int r = 0; r < V; ++r) {
+    const int ci = lane + 64 * r;
+    const bool own = ci < nCore;
+    ownMask[r] = dlp_ballot(own);
+    entered[r] = false;
+    fm[r] = dlp_ballot(own && flags0[ci] != 0);
+    len[r] = own ? (rlL[ci] & kRowLenMask) : 0;
+    l0[r] = own ? lab0L[ci] : 0.0f;
+    l1[r] = own ? lab1L[ci] : 0.0f;
+    outPos[r] = 0u;
+#pragma unroll
+    for (int q = 0; q < V; ++q) out[r][q] = 0ull;
+#pragma unroll
+    for (int p = 0; p < L; ++p) {
+      const int2 rec = own ? recS[ci * RS + p] : make_int2(nullSlot, 0);
+      src[r][p] = rec.x;
+      w[r][p] = __int_as_float(rec.y);
+      if (rec.x < nCore) {
+#pragma unroll
+        for (int q = 0; q < V; ++q)
+          if ((rec.x >> 6) == q) out[r][q] |= 1ull << (rec.x & 63);
+      } else if (rec.x < nCore + nHE) {
+        outPos[r] |= 1u << p;
+      }
+    }
+  }
+  __syncthreads();  // the flags are read
+
+  int reason = 0;
+  unsigned long long ownerSum = 0ull;
+  while (true) {
+    int n = 0;
+#pragma unroll
+    for (int r = 0; r < V; ++r) n += __popcll(fm[r]);
+    if (k >= maxIter) {  // uniform: the size of frontier k decides the reason
+      reason = n ? DLP_LOCAL_CAPPED : DLP_LOCAL_CONVERGED;
+      break;
+    }
+    if (n == 0) {
+      reason = DLP_LOCAL_CONVERGED;
+      break;
+    }
+    const int kb = k & 1;
+    const float *Fo = kb ? lab1L : lab0L;
+    float *Fn = kb ? lab0L : lab1L;
+    unsigned long long cm[V];
+    bool ch[V], anyOut = false;
+#pragma unroll
+    for (int r = 0; r < V; ++r) {
+      const bool f = (fm[r] >> lane) & 1ull;
+      const float old = kb ? l1[r] : l0[r];
+      const float nl = mask_row_label_any<L>(src[r], w[r], Fo, nullSlot, f, len[r], old);
+      ch[r] = f && fabsf(nl - old) > a.delta;
+      *(f ? &Fn[lane + 64 * r] : (float *)dumOut) = nl;
+      l0[r] = (f && kb) ? nl : l0[r];
+      l1[r] = (f && !kb) ? nl : l1[r];
+      cm[r] = dlp_ballot(ch[r]);
+      prof.E += f ? (unsigned)len[r] : 0u;
+      prof.C += ch[r] ? (unsigned)len[r] + 1u : 0u;
+      anyOut = anyOut || (ch[r] && outPos[r] != 0u);
+      entered[r] = entered[r] || (f && k > kBuild);
+    }
+    const bool rebuild = __any(anyOut);  // uniform
+    if (rebuild) {  // marks of eligible vertices outside the core
+      const LocalIter it{stampL, nullptr, nullptr, sh, haloE, nullptr, dumOut, nCore, nHE,
+                         k + 1, stampBase + k + 1, stampBase};
+#pragma unroll
+      for (int r = 0; r < V; ++r) {
+        if (!(ch[r] && outPos[r])) continue;
+#pragma unroll
+        for (int p = 0; p < L; ++p)
+          if ((outPos[r] >> p) & 1u) prof.A += mark_outside(a, l, it, haloE[src[r][p] - nCore]);
+      }
+    }
+    /* next frontier: every changed vertex and the core vertices their rows
+     * name */
+    if (symmetric) {
+#pragma unroll
+      for (int r = 0; r < V; ++r) {
+        unsigned long long meet = 0ull;
+#pragma unroll
+        for (int q = 0; q < V; ++q) meet |= out[r][q] & cm[q];
+        fm[r] = dlp_ballot(ch[r] || meet != 0ull) & ownMask[r];
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < V; ++q) {
+        unsigned long long reach = 0ull;
+#pragma unroll
+        for (int r = 0; r < V; ++r) reach |= ch[r] ? out[r][q] : 0ull;
+        fm[q] = (wave_or64(reach) | cm[q]) & ownMask[q];
+      }
+    }
+    if (lane == 0) {
+      if (k == 0) c->frontier0 = n;
+      if (a.iterSizes && k < a.iterSizesCap) a.iterSizes[k] = n;
+    }
+    ownerSum += (unsigned long long)n;
+    ++k;
+    __syncthreads();  // new labels (and mark_outside's writes) visible to every lane
+    if (rebuild) break;
+  }
+
+  /* the masks of iteration k -> its queue (by core index); stamps of the
+   * vertices that entered a frontier */
+  {
+    int *cur = (k & 1) ? q1L : q0L;
+    int base = 0, nOut = 0;
+#pragma unroll
+    for (int r = 0; r < V; ++r) nOut += __popcll(fm[r]);
+#pragma unroll
+    for (int r = 0; r < V; ++r) {
+      const bool f = (fm[r] >> lane) & 1ull;
+      entered[r] = entered[r] || (f && k > kBuild);
+      if (f) cur[base + dlp_rank_below(fm[r])] = lane + 64 * r;
+      base += __popcll(fm[r]);
+      if (entered[r]) stampL[lane + 64 * r] = k;  // k > kBuild
+    }
+    if (lane == 0) {
+      sh[SH_QC0 + k % 3] = nOut;
+      passUpdates += ownerSum;
+      prof.U += (unsigned)ownerSum;
+      prof.D += (unsigned)ownerSum;
+      prof.Wv += (unsigned)ownerSum;
+      if (ownerSum > 0ull) prof.A += (unsigned)(ownerSum - (unsigned long long)nq) + (unsigned)nOut;
+    }
+    __syncthreads();
+  }
+  return reason;
+}
+
+/* Register-mask owner mode with W = LT / 64 waves (2 .. kMaskWavesMax):
+ * thread tid owns core vertex tid (core at most LT vertices, rows at most
+ * L <= SR entries, every core vertex eligible) and keeps in registers its
+ * row records, both labels and, as W 64-bit masks, the core vertices its
+ * row names.  Each wave holds its part of the frontier as one 64-bit mask.
+ * An iteration: flagged threads compute their labels (neighbour labels from
+ * the LDS label array) and store them; every wave publishes its changed
+ * mask, and whether a change reached an eligible vertex outside the core,
+ * in LDS; one barrier; then each thread's next flag is `changed`, or its row
+ * mask meeting some wave's changed mask (the pipeline's CSR is symmetric;
+ * a caller's CSR -- status array given -- never comes here).  The size of
+ * the next frontier is published with the masks of the following
+ * iteration, so one barrier per iteration suffices: an iteration whose
+ * frontier turns out empty has run with no flagged thread and changed
+ * nothing.  State left behind as owner_iterations. */
+template <int LT, int SR, int L>
+__device__ __forceinline__ int mask_owner_iterations(
+    const PropArgs &a, const LocalArgs &l, LocalShared &S, int &k, int maxIter, int kBuild,
+    int nCore, int nHE, int nullSlot, int stampBase, const int *rlL, const int2 *recS,
+    int *stampL, int *q0L, int *q1L, float *lab0L, float *lab1L, const int *haloE,
+    unsigned long long &passUpdates, LocalProf &prof) {
+  constexpr int W = LT / 64;
+  static_assert(W >= 2 && W <= kMaskWavesMax && L <= SR, "mask owner shape");
+  constexpr int RS = SR + 1;
+  int *sh = S.sh;
+  DevCounters *c = a.ctr;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  int *dumOut = S.dummyOut + lane;
+  const int nq = sh[SH_QC0 + k % 3];
+  {
+    const int *qin = (k & 1) ? q1L : q0L;
+    int *flags = (k & 1) ? q0L : q1L;
+    for (int i = tid; i < nCore; i += LT) flags[i] = 0;
+    __syncthreads();
+    for (int i = tid; i < nq; i += LT) flags[qin[i]] = 1;
+    __syncthreads();
+  }
+  const int *flags0 = (k & 1) ? q0L : q1L;
+  const bool own = tid < nCore;
+  const unsigned long long ownMask = dlp_ballot(own);
+  unsigned long long fm = dlp_ballot(own && flags0[tid] != 0);
+  int src[L];
+  float w[L];
+  unsigned long long out[W];
+  const int len = own ? (rlL[tid] & kRowLenMask) : 0;
+  float l0 = own ? lab0L[tid] : 0.0f, l1 = own ? lab1L[tid] : 0.0f;
+  unsigned outPos = 0u;
+  bool entered = false;
+#pragma unroll
+  for (int q = 0; q < W; ++q) out[q] = 0ull;
+#pragma unroll
+  for (int p = 0; p < L; ++p) {
+    const int2 rec = own ? recS[tid * RS + p] : make_int2(nullSlot, 0);
+    src[p] = rec.x;
+    w[p] = __int_as_float(rec.y);
+    if (rec.x < nCore) {
+#pragma unroll
+      for (int q = 0; q < W; ++q)
+        if ((rec.x >> 6) == q) out[q] |= 1ull << (rec.x & 63);
+    } else if (rec.x < nCore + nHE) {
+      outPos |= 1u << p;
+    }
+  }
+  int n = nq;  // size of frontier k, known for the first iteration
+  __syncthreads();  // the flags are read
+
+  int reason = 0;
+  unsigned long long ownerSum = 0ull;
+  bool first = true;
+  while (true) {
+    if (k >= maxIter) {
+      if (!first) {  // the size of frontier k was published in the last iteration
+        __syncthreads();
+        n = 0;
+#pragma unroll
+        for (int q = 0; q < W; ++q) n += S.xCount[(k - 1) & 1][q];
+      }
+      reason = n ? DLP_LOCAL_CAPPED : DLP_LOCAL_CONVERGED;
+      break;
+    }
+    const int kb = k & 1;
+    const float *Fo = kb ? lab1L : lab0L;
+    float *Fn = kb ? lab0L : lab1L;
+    const bool f = (fm >> lane) & 1ull;
+    const float old = kb ? l1 : l0;
+    float nl = old;
+    if (fm != 0ull)  // wave-uniform: a wave with no flagged vertex only syncs
+      nl = mask_row_label_any<L>(src, w, Fo, nullSlot, f, len, old);
+    const bool ch = f && fabsf(nl - old) > a.delta;
+    *(f ? &Fn[tid] : (float *)dumOut) = nl;
+    l0 = (f && kb) ? nl : l0;
+    l1 = (f && !kb) ? nl : l1;
+    const unsigned long long cmW = dlp_ballot(ch);
+    const bool outW = __any(ch && outPos != 0u);
+    if (outW) {  // wave-uniform: marks of eligible vertices outside the core
+      const LocalIter it{stampL, nullptr, nullptr, sh, haloE, nullptr, dumOut, nCore, nHE,
+                         k + 1, stampBase + k + 1, stampBase};
+      if (ch && outPos) {
+#pragma unroll
+        for (int p = 0; p < L; ++p)
+          if ((outPos >> p) & 1u) prof.A += mark_outside(a, l, it, haloE[src[p] - nCore]);
+      }
+    }
+    if (lane == 0) {
+      S.xChanged[kb][wave] = cmW;
+      S.xOut[kb][wave] = outW ? 1u : 0u;
+    }
+    __syncthreads();
+    if (!first) {  // the size of frontier k, published in iteration k - 1
+      int cnt[W];
+#pragma unroll
+      for (int q = 0; q < W; ++q) cnt[q] = S.xCount[kb ^ 1][q];
+      n = 0;
+#pragma unroll
+      for (int q = 0; q < W; ++q) n += cnt[q];
+    }
+    first = false;
+    if (n == 0) {  // frontier k was empty: nothing above changed anything
+      reason = DLP_LOCAL_CONVERGED;
+      break;
+    }
+    prof.E += f ? (unsigned)len : 0u;
+    prof.C += ch ? (unsigned)len + 1u : 0u;
+    entered = entered || (f && k > kBuild);
+    /* every wave's words first (independent loads), then branch-free ORs */
+    int outs = 0;
+#pragma unroll
+    for (int q = 0; q < W; ++q) outs |= S.xOut[kb][q];
+    unsigned long long xc[W], meet = 0ull;
+#pragma unroll
+    for (int q = 0; q < W; ++q) xc[q] = S.xChanged[kb][q];
+#pragma unroll
+    for (int q = 0; q < W; ++q) meet |= out[q] & xc[q];
+    const bool rebuild = outs != 0;
+    fm = dlp_ballot(ch || meet != 0ull) & ownMask;
+    if (lane == 0) S.xCount[kb][wave] = (unsigned char)__popcll(fm);
+    if (tid == 0) {
+      if (k == 0) c->frontier0 = n;
+      if (a.iterSizes && k < a.iterSizesCap) a.iterSizes[k] = n;
+    }
+    ownerSum += (unsigned long long)n;
+    ++k;
+    if (rebuild) break;  // uniform
+  }
+
+  /* the masks of iteration k -> its queue (by core index); stamps of the
+   * vertices that entered a frontier */
+  {
+    int *cur = (k & 1) ? q1L : q0L;
+    const bool f = (fm >> lane) & 1ull;
+    entered = entered || (f && k > kBuild);
+    __syncthreads();  // every wave's mask of iteration k is final
+    if (lane == 0) S.xCount[0][wave] = (unsigned char)__popcll(fm);
+    __syncthreads();
+    int base = 0, nOut = 0;
+#pragma unroll
+    for (int q = 0; q < W; ++q) {
+      base += q < wave ? S.xCount[0][q] : 0;
+      nOut += S.xCount[0][q];
+    }
+    if (f) cur[base + dlp_rank_below(fm)] = tid;
+    if (entered) stampL[tid] = k;  // k > kBuild
+    if (tid == 0) {
+      sh[SH_QC0 + k % 3] = nOut;
+      passUpdates += ownerSum;
+      prof.U += (unsigned)ownerSum;
+      prof.D += (unsigned)ownerSum;
+      prof.Wv += (unsigned)ownerSum;
+      if (ownerSum > 0ull) prof.A += (unsigned)(ownerSum - (unsigned long long)nq) + (unsigned)nOut;
+    }
+    __syncthreads();
+  }
+  return reason;
+}
+
+/* the multi-wave register-mask owner iterations for the core's longest row */
+template <int LT, int SR>
+__device__ __forceinline__ int mask_owner_dispatch(
+    const PropArgs &a, const LocalArgs &l, LocalShared &S, int &k, int maxIter, int kBuild,
+    int nCore, int nHE, int nullSlot, int stampBase, const int *rlL, const int2 *recS,
+    int *stampL, int *q0L, int *q1L, float *lab0L, float *lab1L, const int *haloE,
+    unsigned long long &passUpdates, LocalProf &prof, int maxLen) {
+  if (maxLen <= 8)  // 8-record rows in registers: fewer registers
+    return mask_owner_iterations<LT, SR, 8>(a, l, S, k, maxIter, kBuild, nCore, nHE, nullSlot,
+                                            stampBase, rlL, recS, stampL, q0L, q1L, lab0L, lab1L,
+                                            haloE, passUpdates, prof);
+  return mask_owner_iterations<LT, SR, SR>(a, l, S, k, maxIter, kBuild, nCore, nHE, nullSlot,
+                                           stampBase, rlL, recS, stampL, q0L, q1L, lab0L, lab1L,
+                                           haloE, passUpdates, prof);
+}
+
+/* the single-wave owner iterations for the core's size and longest row */
+template <int SR>
+__device__ __forceinline__ int wave_owner_dispatch(
+    const PropArgs &a, const LocalArgs &l, LocalShared &S, int &k, int maxIter, int kBuild,
+    int nCore, int nHE, int nullSlot, int stampBase, const int *rlL, const int2 *recS,
+    int *stampL, int *q0L, int *q1L, float *lab0L, float *lab1L, const int *haloE,
+    unsigned long long &passUpdates, LocalProf &prof, int maxLen) {
+#define DLP_WAVE_OWNER(V, LL)                                                                  \
+  return wave_owner_iterations<SR, V, LL>(a, l, S, k, maxIter, kBuild, nCore, nHE, nullSlot,  \
+                                          stampBase, rlL, recS, stampL, q0L, q1L, lab0L, lab1L, \
+                                          haloE, passUpdates, prof)
+  (void)nCore;  // at most 64: one vertex per lane
+  if (maxLen <= 8) DLP_WAVE_OWNER(1, 8);
+  DLP_WAVE_OWNER(1, SR);
+#undef DLP_WAVE_OWNER
+}
+
 /* Runs LDS-resident passes from iteration k (updated) until the frontier
  * empties, k reaches maxIter, or the working set outgrows the LDS budget;
  * returns the DLP_LOCAL_* reason.  All LT threads of the work-group call
  * it; `pool` is the dynamic LDS (l.ldsWords words). */
-template <int LT, int SR, bool OWNER>
+template <int LT, int SR, bool OWNER, bool MASK = false>
 __device__ __forceinline__ int local_passes(const PropArgs &a, const LocalArgs &l, int &k, int maxIter,
                             int *pool, LocalShared &S, LocalProf &prof, int longAfter) {
   static_assert(SR <= kLaneRowMax, "padded stride above the lane row limit");
@@ -1026,7 +1474,29 @@ __device__ __forceinline__ int local_passes(const PropArgs &a, const LocalArgs &
      * otherwise cost the queue iterations spills. */
     constexpr int OV = 2048 / LT;  // owned vertices per thread in owner mode: cores up to 2048
     const bool ownerOK = OWNER && nCore <= OV * LT && !sh[SH_INELIG];  // uniform
-    if (ownerOK) {
+    if constexpr (OWNER && MASK) {
+      /* the register-mask owner kernels: one core vertex per thread (cores
+       * of at most LT vertices: with more per thread the serial work
+       * outweighs the barriers saved), rows of at most SR entries, a CSR
+       * of the pipeline (symmetric); anything else goes back to the host
+       * for a larger mask kernel or the flag-based owner kernel */
+      if (!ownerOK || nCore > LT || hasLong || Elong > 0 || (LT > 64 && a.status)) {
+        reason = DLP_LOCAL_WIDE;
+      } else {
+        if (tid == 0) sh[SH_MAXLEN] = 0;
+        __syncthreads();
+        for (int i = tid; i < nCore; i += LT) atomicMax(&sh[SH_MAXLEN], rlL[i] & kRowLenMask);
+        __syncthreads();
+        if constexpr (LT == 64)
+          reason = wave_owner_dispatch<SR>(a, l, S, k, maxIter, kBuild, nCore, nHE, nullSlot,
+                                           stampBase, rlL, recS, stampL, q0L, q1L, lab0L, lab1L,
+                                           haloE, passUpdates, prof, sh[SH_MAXLEN]);
+        else
+          reason = mask_owner_dispatch<LT, SR>(a, l, S, k, maxIter, kBuild, nCore, nHE, nullSlot,
+                                               stampBase, rlL, recS, stampL, q0L, q1L, lab0L,
+                                               lab1L, haloE, passUpdates, prof, sh[SH_MAXLEN]);
+      }
+    } else if (ownerOK) {
       if (nCore <= LT)
         reason = owner_iterations<LT, SR, 1>(a, l, S, k, maxIter, kBuild, nCore, nHE, nullSlot,
                                              stampBase, rlL, rsL, recS, recL, stampL, q0L, q1L,
@@ -1213,10 +1683,12 @@ __device__ __forceinline__ void local_prof_flush(const PropArgs &a, LocalProf &p
   if (threadIdx.x == 0 && prof.U) atomicAdd(&c->profLocalUpdates, (unsigned long long)prof.U);
   if (threadIdx.x == 0 && prof.D)
     atomicAdd(&c->profLocalOwnerUpdates, (unsigned long long)prof.D);
+  if (threadIdx.x == 0 && prof.Wv)
+    atomicAdd(&c->profLocalWaveUpdates, (unsigned long long)prof.Wv);
   prof = LocalProf{};
 }
 
-template <int LT, int SR, bool OWNER>
+template <int LT, int SR, bool OWNER, bool MASK = false>
 __global__ __launch_bounds__(LT) void k_propagate_local(PropArgs a, LocalArgs l, int k0,
                                                         int maxIter) {
   extern __shared__ int4 pool4[];
@@ -1224,7 +1696,7 @@ __global__ __launch_bounds__(LT) void k_propagate_local(PropArgs a, LocalArgs l,
   if (threadIdx.x < 64) S.dummyStamp[threadIdx.x] = DLP_INT_MAX;  // first pass starts with a barrier
   LocalProf prof;
   int k = k0;
-  const int reason = local_passes<LT, SR, OWNER>(a, l, k, maxIter, (int *)pool4, S, prof,
+  const int reason = local_passes<LT, SR, OWNER, MASK>(a, l, k, maxIter, (int *)pool4, S, prof,
                                                  OWNER ? -1 : l.ownerAfter);
   if (threadIdx.x == 0) {
     a.ctr->nextIter = k;
@@ -1537,6 +2009,13 @@ int dlp_local_lds_bytes(int *bytes) {
                               hipFuncAttributeMaxDynamicSharedMemorySize, v));
   DLP_LOCAL_SHAPES(DLP_SET_LDS)
 #undef DLP_SET_LDS
+  DLP_HIP(hipFuncSetAttribute((const void *)k_propagate_local<64, 16, true, true>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, v));
+  DLP_HIP(hipFuncSetAttribute((const void *)k_propagate_local<256, 16, true, true>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, v));
+  DLP_HIP(hipFuncSetAttribute((const void *)k_propagate_local<512, 16, true, true>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, v));
+
   *bytes = v;
   return DLP_OK;
 }
@@ -1544,6 +2023,28 @@ int dlp_local_lds_bytes(int *bytes) {
 int dlp_launch_propagate_local(const PropArgs &a, const LocalArgs &l, int k, int maxIter,
                                bool owner, hipStream_t s) {
   const size_t lds = (size_t)l.ldsWords * 4;
+  if (l.shape >= kLocalMaskShape && l.shape < kLocalMaskShape + kMaskLevels) {
+    if (!owner) {
+      dlp_set_error_("the register-mask local kernels run owner mode only");
+      return DLP_ERR_INVALID;
+    }
+    switch (l.shape - kLocalMaskShape) {
+      case 0:
+        hipLaunchKernelGGL((k_propagate_local<64, 16, true, true>), dim3(1), dim3(64), lds, s, a,
+                           l, k, maxIter);
+        break;
+      case 1:
+        hipLaunchKernelGGL((k_propagate_local<256, 16, true, true>), dim3(1), dim3(256), lds, s,
+                           a, l, k, maxIter);
+        break;
+      default:
+        hipLaunchKernelGGL((k_propagate_local<512, 16, true, true>), dim3(1), dim3(512), lds, s,
+                           a, l, k, maxIter);
+        break;
+    }
+    DLP_LAUNCHED("k_propagate_local (register-mask owner)");
+    return DLP_OK;
+  }
   switch (l.shape) {
 #define DLP_LAUNCH_LOCAL(I, T, R)                                                       \
   case I:                                                                              \

@@ -375,14 +375,16 @@ class Context:
         return b.value
 
     def set_tuning(self, groupLanes=0, blocks=0, chunk=0, localThreshold=0, localLdsBytes=0,
-                   localShape=0, localOwnerAfter=0, waveRowMin=0, blockRowMin=0):
+                   localShape=0, localOwnerAfter=0, waveRowMin=0, blockRowMin=0,
+                   maskOwnerMax=0):
         """0 keeps a setting; localThreshold < 0 disables the LDS-resident
         propagation kernel; localShape is 1-based; localOwnerAfter -1 runs
         the LDS-resident kernel with queue iterations only; rows longer than
         waveRowMin / blockRowMin go to a wave / a work-group (blockRowMin -1:
-        never)."""
+        never); owner mode on frontiers of at most maskOwnerMax vertices runs in
+        the register-mask kernels (-1: never)."""
         t = _lib.Tuning(groupLanes, blocks, chunk, localThreshold, localLdsBytes, localShape,
-                        localOwnerAfter, waveRowMin, blockRowMin)
+                        localOwnerAfter, waveRowMin, blockRowMin, maskOwnerMax)
         check(self._lib.dlp_context_set_tuning(self._h, C.byref(t)))
 
     def tuning(self) -> dict:

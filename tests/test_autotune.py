@@ -43,7 +43,7 @@ def test_heuristic_turns_the_workgroup_tier_on_for_hubs():
 
 def test_tuning_table_round_trip(tmp_path):
     text = ("# a table\n"
-            "context 19 7 5 : 8 0 1024 1017 0 3 1024 64 -1 ms=412.500\n"
+            "context 19 7 5 : 8 0 1024 1017 0 3 1024 64 -1 512 ms=412.500\n"
             "multi 8 24 : 1 100 8 1 16384 ms=19000.000\n")
     src = tmp_path / "in.tune"
     src.write_text(text)

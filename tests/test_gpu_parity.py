@@ -171,6 +171,9 @@ def test_engine_mix_long_rows(gpu_available, shape, owner):
                           localOwnerAfter=owner)
     assert prof["localLaunches"] > 0
     assert (prof["localOwnerUpdates"] > 0) == (owner > 0)
+    # every row is longer than 16 entries: the single-wave owner kernel hands
+    # each core back to the work-group owner kernel
+    assert prof["localWaveUpdates"] == 0
 
 
 @pytest.mark.parametrize("shape", [1, 2, 3, 4, 5])
@@ -192,7 +195,9 @@ def test_owner_mode_and_queue_iterations(gpu_available, c1_inputs, shape, owner,
         assert prof["localOwnerUpdates"] == 0
     elif owner == 1:
         assert prof["localOwnerUpdates"] > 0
-    assert prof["localOwnerUpdates"] <= prof["localUpdates"]
+        # small cores run in the single-wave owner kernel
+        assert prof["localWaveUpdates"] > 0
+    assert prof["localWaveUpdates"] <= prof["localOwnerUpdates"] <= prof["localUpdates"]
 
 
 def hub_inputs(tmp_path, hubs=((0, 700), (1, 320))):
