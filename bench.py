@@ -372,13 +372,16 @@ def serve_run(m, tenants, reqTenant, arrivals, plain, args, keep_labels=False):
     m.synchronize()
     reqs = list(zip(reqTenant.tolist(), arrivals.tolist()))
     done, steps = m.serve(reqs, maxSlots=args.slots, hbmBudgetBytes=args.hbm_budget,
-                          plain=plain, handBackLanes=args.lanes_back)
+                          plain=plain, handBackLanes=args.lanes_back,
+                          sliceIterations=args.slice_iterations, sliceMs=args.slice_ms)
     m.synchronize()
     lat = 1000.0 * (done - arrivals)
     assert np.all(lat >= 0.0), "a request completed before it arrived"
     upd = sum(m.stats(t, b)["updates"] for t in range(T)
               for b in range(1, tenants[t].numBatches))
+    pool = m.pool_stats()
     out = {"makespan_s": float(done.max()), "steps": steps, "updates": int(upd),
+           "pool_peak_bytes": pool["peak"], "pool_deferrals": pool["deferrals"],
            "updates_per_s": upd / float(done.max()),
            "min_ms": float(lat.min()), "p50_ms": float(np.percentile(lat, 50)),
            "p99_ms": float(np.percentile(lat, 99))}
@@ -443,6 +446,7 @@ def serving_leg(args, progress, sweep_rates) -> dict:
                     f"{int(args.max_tenant_nodes)}], degree 5, tenant t seed 1042+t; {R} "
                     "batches of log-uniform 100-100000 vertices (90/1/9)",
         "slots": args.slots, "hbm_budget_bytes": args.hbm_budget,
+        "slice_iterations": args.slice_iterations, "slice_ms": args.slice_ms,
         "updates": sat_served["updates"],
         "saturated": {"served": sat_served, "plain": sat_plain,
                       "speedup_vs_plain": sat_served["updates_per_s"] /
@@ -708,7 +712,14 @@ def main() -> int:
                     help="largest tenant graph (serving)")
     ap.add_argument("--slots", type=int, default=64, help="batches per step (serving)")
     ap.add_argument("--hbm-budget", type=int, default=8 << 30,
-                    help="HBM working-set budget of one step in bytes (serving)")
+                    help="serving: bytes of the scratch pool the running batches draw "
+                         "their per-batch scratch from (0 = per-tenant scratch)")
+    ap.add_argument("--slice-iterations", type=int, default=0,
+                    help="serving: iterations a batch advances per step (0 = library "
+                         "default, -1 = to the end)")
+    ap.add_argument("--slice-ms", type=float, default=0.0,
+                    help="serving: ms of HBM iterations per step (0 = library default, "
+                         "< 0 = no limit)")
     ap.add_argument("--rates", type=float, nargs="*", default=[0.5, 1.0, 2.0, 4.0],
                     help="arrival rates as multiples of the plain path's capacity (serving)")
     ap.add_argument("--graphs", type=int, default=256, help="graphs of the batched workload")

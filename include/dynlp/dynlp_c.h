@@ -459,6 +459,28 @@ int dlp_multi_batch_stats(dlp_multi *m, int graph, int b, dlp_batch_stats *out);
 int dlp_multi_batch_time(dlp_multi *m, int b, float *ms);
 int dlp_multi_device_bytes(const dlp_multi *m, int64_t *bytes);
 
+/* Scratch pool (no counterpart in the reference, which runs one graph).
+ * poolBytes > 0: the graphs give up their own per-batch scratch (frontier
+ * queues, candidate and touched lists, the sparsified subgraph, sort and
+ * reduction buffers) and every batch draws it, while it runs, from one
+ * device allocation of poolBytes shared by all graphs: dlp_multi_serve
+ * admits a batch only when its scratch fits the pool's free space (else it
+ * is deferred), and dlp_multi_run_batch / run_next fail with DLP_ERR_NOMEM
+ * when the step's batches do not fit at once.  0 = every graph its own
+ * scratch again (the default).  Graph state is kept either way. */
+int dlp_multi_set_scratch_pool(dlp_multi *m, int64_t poolBytes);
+typedef struct dlp_pool_stats {
+  int64_t capacity;  /* bytes of the pool; 0 = per-graph scratch          */
+  int64_t inUse;     /* bytes lent to running batches now                 */
+  int64_t peak;      /* most bytes lent at once since the pool was set    */
+  int64_t leases;    /* batches that drew their scratch from the pool     */
+  int64_t deferrals; /* admissions dlp_multi_serve deferred for lack of
+                        pool space                                        */
+} dlp_pool_stats;
+int dlp_multi_pool_stats(dlp_multi *m, dlp_pool_stats *out);
+/* bytes of scratch batch b of graph g draws from a pool */
+int dlp_multi_batch_scratch_bytes(const dlp_multi *m, int graph, int b, int64_t *bytes);
+
 /* dlp_context_set_counter_bases / dlp_context_counter_bases for graph g. */
 int dlp_multi_set_counter_bases(dlp_multi *m, int graph, int stampBase, int localGen);
 int dlp_multi_counter_bases(dlp_multi *m, int graph, int *stampBase, int *localGen);
@@ -473,24 +495,36 @@ typedef struct dlp_serve_request {
 } dlp_serve_request;
 
 typedef struct dlp_serve_config {
-  int maxSlots;           /* batches admitted into one step (>= 1)    */
-  int64_t hbmBudgetBytes; /* HBM working set of the admitted batches
-                             (lists, G', sort scratch); 0 = unlimited  */
-  int plain;              /* 1: one batch per step, single-graph path  */
+  int maxSlots;           /* batches in the shared launch (>= 1)       */
+  int64_t hbmBudgetBytes; /* > 0: size of the scratch pool the running
+                             batches draw from (dlp_multi_set_scratch_
+                             pool); 0 = the runner's current scratch   */
+  int plain;              /* 1: one batch at a time, single-graph path */
   int handBackLanes;      /* host threads (each with a stream) that
                              finish handed-back graphs (multi tuning
                              heavyFrontier > 0) beside later steps;
                              0 = 8                                     */
+  int sliceIterations;    /* a step stops each graph after this many
+                             iterations (0 = 2048, -1 = no limit) ...  */
+  double sliceMs;         /* ... or after this many milliseconds of its
+                             work-group HBM iterations (0 = 2.0, < 0 =
+                             no limit); the next step resumes it       */
 } dlp_serve_config;
 
-/* Arrival-driven continuous batching: requests (sorted by arrival) are
- * released at their arrival times; each step admits, in arrival order,
- * at most one batch per graph, up to maxSlots batches and the HBM budget,
- * and runs them together; completion[i] receives the time (seconds from
- * the start) at which request i's labels were ready.  With hand-back on,
- * a graph whose frontier outgrows heavyFrontier leaves its step and is
- * finished by a hand-back lane while the next steps run; its tenant's
- * next request waits for it. */
+/* Arrival-driven continuous batching with iteration-level admission:
+ * requests (sorted by arrival) are released at their arrival times.  A
+ * step is one shared propagation launch in which every running batch
+ * advances by at most one slice (sliceIterations / sliceMs) and stops at
+ * an iteration boundary; before each step, arrived requests are admitted
+ * in arrival order -- at most one batch per graph, up to maxSlots running
+ * batches, and only when the batch's scratch fits the pool (else it is
+ * deferred; a batch larger than the whole pool is an error, DLP_ERR_NOMEM)
+ * -- and join the running ones.  completion[i] receives the time (seconds
+ * from the start) at which request i's labels were ready.  With hand-back
+ * on, a graph whose frontier outgrows heavyFrontier leaves the shared
+ * launches and is finished by a hand-back lane while the steps go on; its
+ * tenant's next request waits for it.  Stopping and resuming a graph does
+ * not change its iterations or labels. */
 int dlp_multi_serve(dlp_multi *m, const dlp_serve_request *reqs, int numReqs,
                     const dlp_serve_config *cfg, double *completion, int *steps);
 

@@ -142,7 +142,13 @@ class ServeRequest(C.Structure):
 
 class ServeConfig(C.Structure):
     _fields_ = [("maxSlots", C.c_int), ("hbmBudgetBytes", C.c_int64), ("plain", C.c_int),
-                ("handBackLanes", C.c_int)]
+                ("handBackLanes", C.c_int), ("sliceIterations", C.c_int),
+                ("sliceMs", C.c_double)]
+
+
+class PoolStats(C.Structure):
+    _fields_ = [("capacity", C.c_int64), ("inUse", C.c_int64), ("peak", C.c_int64),
+                ("leases", C.c_int64), ("deferrals", C.c_int64)]
 
 
 class PropProfile(C.Structure):
@@ -257,6 +263,9 @@ SIGNATURES = {
     "dlp_multi_batch_time": (C.c_int, [_VP, C.c_int, C.POINTER(C.c_float)]),
     "dlp_multi_device_bytes": (C.c_int, [_VP, C.POINTER(C.c_int64)]),
     "dlp_multi_run_next": (C.c_int, [_VP, C.POINTER(C.c_int), C.c_int]),
+    "dlp_multi_set_scratch_pool": (C.c_int, [_VP, C.c_int64]),
+    "dlp_multi_pool_stats": (C.c_int, [_VP, C.POINTER(PoolStats)]),
+    "dlp_multi_batch_scratch_bytes": (C.c_int, [_VP, C.c_int, C.c_int, C.POINTER(C.c_int64)]),
     "dlp_multi_serve": (C.c_int, [_VP, C.POINTER(ServeRequest), C.c_int, C.POINTER(ServeConfig),
                                   C.POINTER(C.c_double), C.POINTER(C.c_int)]),
     "dlp_inputs_generate_stream": (C.c_int, [C.POINTER(_VP), C.c_int, C.c_int,

@@ -40,6 +40,12 @@ struct dlp_context {
                     kDefaultBlockRowMin, kMaskOwnerMax};
   LocalArgs local{};
   int localMaxBytes = 0;
+  /* per-batch scratch (queues, candidate / touched lists, G', sort and
+   * reduction buffers): the context's own allocation, sized for its largest
+   * batch, or none while a runner's scratch pool lends it per batch */
+  void *scratchAlloc = nullptr;
+  int64_t scratchAllocBytes = 0;
+  int maxIns = 0, maxE = 0;
   int nextBatch = 0;
   /* upper bounds of the device's stampBase and localGen (dlp_guard_plan) */
   long long stampBound = 1, genBound = 0;
@@ -83,6 +89,8 @@ struct dlp_context {
     if (stream) (void)hipStreamSynchronize(stream);
     for (void *p : allocs) (void)hipFree(p);
     allocs.clear();
+    if (scratchAlloc) (void)hipFree(scratchAlloc);
+    scratchAlloc = nullptr;
     for (hipEvent_t e : timing.ev) (void)hipEventDestroy(e);
     timing.ev.clear();
     for (hipEvent_t e : evStart) (void)hipEventDestroy(e);
@@ -96,6 +104,18 @@ struct dlp_context {
   }
 };
 
+
+/* Per-batch scratch of a batch with `ins` inserted vertices and `e` new
+ * edges: its size in bytes and, when base is non-null, the per-batch
+ * pointers of c->d set into [base, base + size) (each buffer 256-byte
+ * aligned).  base == nullptr with bind leaves the pointers null. */
+int64_t dlp_ctx_scratch_layout(dlp_context *c, long long ins, long long e, char *base, bool bind);
+/* scratch bytes of batch b */
+int64_t dlp_ctx_batch_scratch_bytes(const dlp_context *c, int b);
+/* own = true: allocate the context's own scratch (its largest batch) and
+ * bind it; false: free it and leave the per-batch pointers null (stream
+ * idle) */
+int dlp_ctx_own_scratch(dlp_context *c, bool own);
 
 /* before a batch of the context: rebases its iteration stamps and restarts
  * its LDS generations on stream s when the batch could wrap them (enqueued

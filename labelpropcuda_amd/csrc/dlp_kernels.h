@@ -161,9 +161,22 @@ struct GraphProp {
   int backoff;      /* HBM iterations after an LDS overflow           */
   int heavy;        /* > 0: a frontier larger than this hands the graph
                        back to the host at that iteration              */
-  int *exitIter;    /* device int: -1 = propagation finished, else the
-                       iteration to resume at with the full-GPU loop   */
+  int startIter;    /* iteration the launch starts the graph at (0 for a
+                       new batch; the queue and counters of startIter
+                       are on the device)                               */
+  int stopIter;     /* the launch stops the graph at this iteration
+                       (<= maxIter) ...                                 */
+  long long sliceTicks; /* ... or, once it ran an iteration, at the first
+                       iteration boundary after this many wall-clock
+                       ticks of the launch (0 = no time limit)          */
+  int *exitIter;    /* device int: -1 = propagation finished (frontier
+                       empty or maxIter reached); k >= 0 = handed back
+                       at iteration k (resume with the full-GPU loop);
+                       -2 - k = stopped at iteration k by stopIter or
+                       sliceTicks (resume in a later launch)            */
 };
+/* exitIter value of a graph stopped at iteration k by its slice */
+__host__ __device__ inline int dlp_slice_exit(int k) { return -2 - k; }
 
 /* Runs the propagation of every graph in `graphs` (device array, count
  * numGraphs) to completion in one launch, one work-group per graph, with

@@ -32,6 +32,7 @@
 #include <vector>
 
 #include "dlp_context.h"
+#include "dlp_pool.h"
 #include "dlp_tune.h"
 
 struct dlp_multi {
@@ -63,6 +64,13 @@ struct dlp_multi {
   std::vector<GraphProp> props;     /* per graph, batch-independent      */
   GraphProp *dStep = nullptr;       /* device, G entries                 */
   GraphProp *hStep = nullptr;       /* pinned staging, G entries         */
+  /* scratch pool: one device allocation that lends each running batch its
+   * per-batch scratch (poolBase null: every graph has its own) */
+  void *poolBase = nullptr;
+  RangePool pool;
+  int64_t deferrals = 0;      /* serving admissions deferred for pool space */
+  std::vector<int64_t> lease; /* per graph: offset of its lease, -1 = none  */
+  int wallRateKHz = 0;        /* device wall clock (slice deadlines)        */
 
   void drop_graphs() {
     for (std::vector<hipGraphExec_t> *v : {&exec, &execPost})
@@ -90,6 +98,8 @@ struct dlp_multi {
     hExit = nullptr;
     for (dlp_context *c : ctx) dlp_context_destroy(c);
     ctx.clear();
+    if (poolBase) (void)hipFree(poolBase);
+    poolBase = nullptr;
     for (hipEvent_t e : evStart) (void)hipEventDestroy(e);
     for (hipEvent_t e : evEnd) (void)hipEventDestroy(e);
     evStart.clear();
@@ -154,6 +164,9 @@ GraphProp graph_prop(const dlp_multi *m, int gi) {
   if (c->N >= (1 << 30)) g.threshold = 0;  // outside-vertex encoding limit
   g.backoff = m->backoff;
   g.heavy = heavy_lockstep(m);
+  g.startIter = 0;
+  g.stopIter = g.maxIter;
+  g.sliceTicks = 0;
   g.exitIter = m->dExit + gi;
   return g;
 }
@@ -385,6 +398,79 @@ int capture(dlp_multi *m, hipGraphExec_t *out, Fn fn) {
   return DLP_OK;
 }
 
+bool pooled(const dlp_multi *m) { return m->poolBase != nullptr; }
+
+/* Lends graph g the scratch of its next batch from the pool (nothing to do
+ * without a pool or with a lease held).  *granted = false, and nothing
+ * changed, when no free range fits now; DLP_ERR_NOMEM when the batch needs
+ * more than the whole pool. */
+int lease_scratch(dlp_multi *m, int g, bool *granted) {
+  *granted = true;
+  if (!pooled(m) || m->lease[g] >= 0) return DLP_OK;
+  dlp_context *c = m->ctx[g];
+  const int b = c->nextBatch;
+  const int64_t need = dlp_ctx_batch_scratch_bytes(c, b);
+  if (RangePool::rounded(need) > m->pool.capacity()) {
+    dlp_set_errorf_("graph %d: batch %d needs %lld bytes of scratch, the pool holds %lld", g, b,
+                    (long long)need, (long long)m->pool.capacity());
+    return DLP_ERR_NOMEM;
+  }
+  int64_t off = 0;
+  if (!m->pool.acquire(need, &off)) {
+    *granted = false;
+    return DLP_OK;
+  }
+  m->lease[g] = off;
+  dlp_ctx_scratch_layout(c, c->insOff[b + 1] - c->insOff[b], c->eOff[b + 1] - c->eOff[b],
+                         (char *)m->poolBase + off, true);
+  return DLP_OK;
+}
+
+/* returns graph g's lease (its batch's work on the device is complete) */
+void return_scratch(dlp_multi *m, int g) {
+  if (!pooled(m) || m->lease[g] < 0) return;
+  m->pool.release(m->lease[g]);
+  m->lease[g] = -1;
+  dlp_ctx_scratch_layout(m->ctx[g], 0, 0, nullptr, true);
+}
+
+/* Leases for every graph of a step, or none: DLP_ERR_NOMEM when the pool
+ * cannot hold them all at once.  The leases go back when the guard ends
+ * (after the stream is idle). */
+class StepLeases {
+ public:
+  explicit StepLeases(dlp_multi *m) : m_(m) {}
+  ~StepLeases() {
+    if (graphs_.empty()) return;
+    (void)hipStreamSynchronize(m_->stream);
+    for (int g : graphs_) return_scratch(m_, g);
+  }
+  int take(const int *graphs, int n) {
+    if (!pooled(m_)) return DLP_OK;
+    for (int i = 0; i < n; ++i) {
+      bool ok = false;
+      const int rc = lease_scratch(m_, graphs[i], &ok);
+      if (rc != DLP_OK) return rc;
+      if (!ok) {
+        dlp_set_errorf_("the scratch pool (%lld bytes, largest free range %lld) cannot hold the "
+                        "batches of these %d graphs at once",
+                        (long long)m_->pool.capacity(), (long long)m_->pool.largestFree(), n);
+        return DLP_ERR_NOMEM;
+      }
+      graphs_.push_back(graphs[i]);
+    }
+    return DLP_OK;
+  }
+
+ private:
+  dlp_multi *m_;
+  std::vector<int> graphs_;
+};
+
+int run_next_impl(dlp_multi *m, const int *graphs, int count, int heavy,
+                  const std::function<void(const std::vector<int> &)> *done,
+                  const std::function<int(int, int)> *handOff = nullptr, bool lockstep = false);
+
 void host_reset(dlp_multi *m) {
   for (dlp_context *c : m->ctx) {
     c->nextBatch = 0;
@@ -475,6 +561,16 @@ extern "C" int dlp_multi_create(dlp_multi **out, int numGraphs, const dlp_config
     m->dExit = (int *)p;
   }
   MTRY(upload_props(m));
+  m->lease.assign(m->G, -1);
+  {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&m->wallRateKHz, hipDeviceAttributeWallClockRate, dev) !=
+            hipSuccess) {
+      (void)hipGetLastError();
+      m->wallRateKHz = 0;  // no time slices
+    }
+  }
   {
     void *p = nullptr;
     if (hipMalloc(&p, sizeof(GraphProp) * m->G) != hipSuccess ||
@@ -595,6 +691,21 @@ extern "C" int dlp_multi_run_batch(dlp_multi *m, int b) {
       return DLP_ERR_STATE;
     }
   int rc;
+  if (pooled(m)) {  // scratch lent per batch: no captured graphs (their pointers would move)
+    std::vector<int> members;
+    for (int g = 0; g < m->G; ++g)
+      if (b < m->ctx[g]->B) members.push_back(g);
+    StepLeases leases(m);
+    if ((rc = leases.take(members.data(), (int)members.size())) != DLP_OK) return rc;
+    DLP_HIP(hipEventRecord(m->evStart[b], m->stream));
+    if ((rc = run_next_impl(m, members.data(), (int)members.size(), heavy_lockstep(m), nullptr,
+                            nullptr, true)) != DLP_OK)
+      return rc;
+    DLP_HIP(hipEventRecord(m->evEnd[b], m->stream));
+    m->ran[b] = 1;
+    m->nextBatch = b + 1;
+    return DLP_OK;
+  }
   /* wrap-around guards: outside the captured graph, decided per run */
   for (int g = 0; g < m->G; ++g)
     if (b < m->ctx[g]->B && (rc = dlp_ctx_counter_guard(m->ctx[g], m->stream)) != DLP_OK)
@@ -813,7 +924,81 @@ extern "C" int dlp_multi_device_bytes(const dlp_multi *m, int64_t *bytes) {
   }
   int64_t t = 0;
   for (dlp_context *c : m->ctx) t += c->deviceBytes;
-  *bytes = t + (int64_t)sizeof(GraphProp) * m->G * m->maxB;
+  *bytes = t + (int64_t)sizeof(GraphProp) * m->G * m->maxB + (m->poolBase ? m->pool.capacity() : 0);
+  return DLP_OK;
+}
+
+extern "C" int dlp_multi_set_scratch_pool(dlp_multi *m, int64_t poolBytes) {
+  if (!m || poolBytes < 0) {
+    dlp_set_error_("dlp_multi_set_scratch_pool: NULL handle or negative size");
+    return DLP_ERR_INVALID;
+  }
+  DLP_HIP(hipStreamSynchronize(m->stream));
+  for (int64_t l : m->lease)
+    if (l >= 0) {
+      dlp_set_error_("dlp_multi_set_scratch_pool: a batch holds a lease");
+      return DLP_ERR_STATE;
+    }
+  int rc = DLP_OK;
+  if (poolBytes > 0) {
+    const int64_t cap = poolBytes / kPoolAlign * kPoolAlign;
+    if (cap <= 0) {
+      dlp_set_errorf_("dlp_multi_set_scratch_pool: %lld bytes is below one %lld-byte range",
+                      (long long)poolBytes, (long long)kPoolAlign);
+      return DLP_ERR_INVALID;
+    }
+    void *p = nullptr;
+    const hipError_t e = hipMalloc(&p, (size_t)cap);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      dlp_set_errorf_("hipMalloc of a %lld-byte scratch pool failed (%s)", (long long)cap,
+                      hipGetErrorString(e));
+      return DLP_ERR_NOMEM;
+    }
+    for (dlp_context *c : m->ctx)
+      if ((rc = dlp_ctx_own_scratch(c, false)) != DLP_OK) {
+        (void)hipFree(p);
+        return rc;
+      }
+    if (m->poolBase) (void)hipFree(m->poolBase);
+    m->poolBase = p;
+    m->pool.reset(cap);
+  } else {
+    for (size_t i = 0; i < m->ctx.size(); ++i)
+      if ((rc = dlp_ctx_own_scratch(m->ctx[i], true)) != DLP_OK) {
+        for (size_t j = 0; j < i; ++j) (void)dlp_ctx_own_scratch(m->ctx[j], false);
+        return rc;
+      }
+    if (m->poolBase) (void)hipFree(m->poolBase);
+    m->poolBase = nullptr;
+    m->pool.reset(0);
+  }
+  m->deferrals = 0;
+  m->drop_graphs();  // captured launches embed the scratch pointers
+  m->exec.assign(m->maxB, nullptr);
+  m->execPost.assign(m->maxB, nullptr);
+  return upload_props(m);
+}
+
+extern "C" int dlp_multi_pool_stats(dlp_multi *m, dlp_pool_stats *out) {
+  if (!m || !out) {
+    dlp_set_error_("dlp_multi_pool_stats: NULL argument");
+    return DLP_ERR_INVALID;
+  }
+  out->capacity = m->poolBase ? m->pool.capacity() : 0;
+  out->inUse = m->pool.inUse();
+  out->peak = m->pool.peak();
+  out->leases = m->pool.leases();
+  out->deferrals = m->deferrals;
+  return DLP_OK;
+}
+
+extern "C" int dlp_multi_batch_scratch_bytes(const dlp_multi *m, int g, int b, int64_t *bytes) {
+  if (!m || !bytes || g < 0 || g >= m->G || b < 0 || b >= m->ctx[g]->B) {
+    dlp_set_error_("dlp_multi_batch_scratch_bytes: bad argument");
+    return DLP_ERR_INVALID;
+  }
+  *bytes = RangePool::rounded(dlp_ctx_batch_scratch_bytes(m->ctx[g], b));
   return DLP_OK;
 }
 
@@ -830,17 +1015,17 @@ namespace {
  * it (graph, resume iteration) instead of being finished here. */
 int run_next_impl(dlp_multi *m, const int *graphs, int count, int heavy,
                   const std::function<void(const std::vector<int> &)> *done,
-                  const std::function<int(int, int)> *handOff = nullptr) {
+                  const std::function<int(int, int)> *handOff, bool lockstep) {
   DLP_HIP(hipStreamSynchronize(m->stream));  // the staging buffer is free
   int rc;
-  m->nextBatch = -1;  // graphs advance one by one: no lockstep batch until a reset
+  if (!lockstep) m->nextBatch = -1;  // graphs advance one by one: no lockstep batch until a reset
   for (int i = 0; i < count; ++i) {
     dlp_context *c = m->ctx[graphs[i]];
     const int b = c->nextBatch;
     int tag = 0;
     if ((rc = dlp_ctx_counter_guard(c, m->stream)) != DLP_OK) return rc;
     if ((rc = dlp_ctx_batch_pre(c, b, dlp_ctx_make_batch(c, b), &tag)) != DLP_OK) return rc;
-    m->hStep[i] = m->props[graphs[i]];
+    m->hStep[i] = graph_prop(m, graphs[i]);  // the scratch pointers of a lease move
     m->hStep[i].heavy = heavy;
   }
   DLP_HIP(hipMemcpyAsync(m->dStep, m->hStep, sizeof(GraphProp) * count, hipMemcpyHostToDevice,
@@ -898,27 +1083,33 @@ extern "C" int dlp_multi_run_next(dlp_multi *m, const int *graphs, int count) {
       return DLP_ERR_STATE;
     }
   }
+  StepLeases leases(m);
+  const int rc = leases.take(graphs, count);
+  if (rc != DLP_OK) return rc;
   return run_next_impl(m, graphs, count, heavy_lockstep(m), nullptr);
 }
 
 namespace {
 
-/* HBM working set of a batch beyond the context's resident state: its
- * lists, the sparsified subgraph and the sort / reduction scratch of its
- * inserted vertices (the admission model of the scheduler) */
-int64_t batch_work_bytes(const dlp_context *c, int b) {
-  const int64_t ins = c->insOff[b + 1] - c->insOff[b];
-  const int64_t del = c->delOff[b + 1] - c->delOff[b];
-  const int64_t e = c->eOff[b + 1] - c->eOff[b];
-  return 64 * ins + 8 * del + 36 * e;
-}
+/* iteration-level admission of dlp_multi_serve: a step stops each graph at
+ * the first iteration boundary after kServeSliceIterations iterations, or
+ * after kServeSliceMs of work-group HBM iterations, and the next step
+ * resumes it beside the batches admitted in between */
+constexpr int kServeSliceIterations = 2048;
+constexpr double kServeSliceMs = 2.0;
+
+/* a batch in the shared propagation launches of dlp_multi_serve */
+struct Flight {
+  int graph, req;
+  int k; /* iteration the next launch starts it at */
+};
 
 }  // namespace
 
 extern "C" int dlp_multi_serve(dlp_multi *m, const dlp_serve_request *reqs, int numReqs,
                                const dlp_serve_config *cfg, double *completion, int *steps) {
   if (!m || numReqs < 0 || (numReqs > 0 && (!reqs || !completion)) || !cfg ||
-      cfg->maxSlots < 1) {
+      cfg->maxSlots < 1 || cfg->hbmBudgetBytes < 0 || cfg->sliceIterations < -1) {
     dlp_set_error_("dlp_multi_serve: bad argument");
     return DLP_ERR_INVALID;
   }
@@ -935,100 +1126,170 @@ extern "C" int dlp_multi_serve(dlp_multi *m, const dlp_serve_request *reqs, int 
       return DLP_ERR_INVALID;
     }
   }
+  int rc;
+  /* the HBM budget is the size of the scratch pool the batches draw from */
+  if (cfg->hbmBudgetBytes > 0 &&
+      (!pooled(m) || m->pool.capacity() != cfg->hbmBudgetBytes / kPoolAlign * kPoolAlign) &&
+      (rc = dlp_multi_set_scratch_pool(m, cfg->hbmBudgetBytes)) != DLP_OK)
+    return rc;
   m->nextBatch = -1;  // graphs advance one by one: no lockstep batch until a reset
   using clock = std::chrono::steady_clock;
   const clock::time_point t0 = clock::now();
   auto now = [&] { return std::chrono::duration<double>(clock::now() - t0).count(); };
   const int slots = cfg->plain ? 1 : cfg->maxSlots;
-  std::vector<int> pending;  // request indices not yet served, in arrival order
+  const int slice = cfg->sliceIterations == 0 ? kServeSliceIterations : cfg->sliceIterations;
+  const double sliceMs = cfg->sliceMs == 0.0 ? kServeSliceMs : cfg->sliceMs;
+  const long long sliceTicks =
+      sliceMs > 0.0 && m->wallRateKHz > 0 ? (long long)(sliceMs * m->wallRateKHz) : 0;
+  std::vector<int> pending;  // request indices not yet admitted, in arrival order
   pending.reserve(numReqs);
-  int nextArrival = 0, nSteps = 0, rc;
-  std::vector<char> blocked(m->G, 0);
-  std::vector<int> admitted, graphs;
-  /* handed-back graphs finish beside the following steps */
+  int nextArrival = 0, nSteps = 0;
+  std::vector<char> blocked(m->G, 0), flying(m->G, 0);
+  std::vector<Flight> flight, fresh, carried;
+  /* handed-back graphs finish beside the following steps; their lease goes
+   * back when they are done */
   std::vector<int> laneReq(m->G, -1);  // request a graph on the lane serves
   std::unique_ptr<HandBackLane> lane;
   const int heavy = cfg->plain ? 0 : heavy_serving(m);
   if (heavy > 0) {
     const int workers = cfg->handBackLanes > 0 ? cfg->handBackLanes : 8;
-    lane.reset(new HandBackLane(m, workers, true,
-                                [&](int g) { completion[laneReq[g]] = now(); }));
+    lane.reset(new HandBackLane(m, workers, true, [&](int g) {
+      completion[laneReq[g]] = now();
+      return_scratch(m, g);
+    }));
     if ((rc = lane->start()) != DLP_OK) return rc;
   }
-  const std::function<int(int, int)> handOff = [&](int g, int k) {
-    for (size_t i = 0; i < graphs.size(); ++i)
-      if (graphs[i] == g) laneReq[g] = admitted[i];
-    return lane->push(g, k);
-  };
   auto laneFailed = [&]() {
     rc = lane ? lane->error() : DLP_OK;
     if (rc != DLP_OK) dlp_set_errorf_("dlp_multi_serve: hand-back lane: %s", lane->message().c_str());
     return rc != DLP_OK;
   };
-  while (nextArrival < numReqs || !pending.empty()) {
+  /* returns every lease still held when an error ends the call */
+  struct LeaseSweep {
+    dlp_multi *m;
+    std::unique_ptr<HandBackLane> *lane;
+    ~LeaseSweep() {
+      if (*lane) (*lane)->stop();
+      (void)hipStreamSynchronize(m->stream);
+      for (int g = 0; g < m->G; ++g) return_scratch(m, g);
+    }
+  } sweep{m, &lane};
+  auto post = [&](int g) {
+    dlp_context *c = m->ctx[g];
+    return dlp_ctx_batch_post(c, c->nextBatch, dlp_ctx_make_batch(c, c->nextBatch));
+  };
+  while (nextArrival < numReqs || !pending.empty() || !flight.empty()) {
     if (laneFailed()) return rc;
     const double t = now();
     while (nextArrival < numReqs && reqs[nextArrival].arrival <= t) pending.push_back(nextArrival++);
     const double nextAt = nextArrival < numReqs ? reqs[nextArrival].arrival : t + 1.0;
-    if (pending.empty()) {
-      std::this_thread::sleep_for(std::chrono::duration<double>(reqs[nextArrival].arrival - t));
-      continue;
-    }
-    /* admission: arrival order, one batch per graph per step, a graph's
-     * batches in order, at most `slots` batches and the HBM budget (the
-     * first admitted batch always fits) */
-    admitted.clear();
-    graphs.clear();
+    /* admission, in arrival order: one batch per graph at a time, a graph's
+     * batches in order, at most `slots` batches in the shared launch, and
+     * scratch from the pool (a batch that does not fit now is deferred; a
+     * later, smaller one may pass it) */
+    fresh.clear();
     std::fill(blocked.begin(), blocked.end(), 0);
-    int64_t used = 0;
-    for (int idx : pending) {
-      const int g = reqs[idx].graph;
-      if (blocked[g]) continue;
-      blocked[g] = 1;  // later requests of g wait for this one
-      if (lane && lane->busy(g)) continue;
-      if ((int)admitted.size() >= slots) continue;
-      const int64_t w = batch_work_bytes(m->ctx[g], m->ctx[g]->nextBatch);
-      if (!admitted.empty() && cfg->hbmBudgetBytes > 0 && used + w > cfg->hbmBudgetBytes)
-        continue;
-      used += w;
-      admitted.push_back(idx);
-      graphs.push_back(g);
-    }
-    if (admitted.empty()) {  // every pending graph is busy on the lane
-      lane->wait(nextAt, now);
-      continue;
-    }
-    if (cfg->plain) {
-      dlp_context *c = m->ctx[graphs[0]];
-      if ((rc = dlp_context_run_batch(c, c->nextBatch)) != DLP_OK) return rc;
-      DLP_HIP(hipStreamSynchronize(c->stream));
-      completion[admitted[0]] = now();
-    } else {
-      /* a request completes when its graph's batch does (a handed-back
-       * graph after the rest of the step) */
-      const std::function<void(const std::vector<int> &)> stamp =
-          [&](const std::vector<int> &done) {
-            const double t = now();
-            for (size_t i = 0; i < graphs.size(); ++i)
-              if (std::find(done.begin(), done.end(), graphs[i]) != done.end())
-                completion[admitted[i]] = t;
-          };
-      if ((rc = run_next_impl(m, graphs.data(), (int)graphs.size(), heavy, &stamp,
-                              lane ? &handOff : nullptr)) != DLP_OK)
-        return rc;
-    }
-    ++nSteps;
     std::vector<int> rest;
     rest.reserve(pending.size());
-    size_t a = 0;
     for (int idx : pending) {
-      if (a < admitted.size() && admitted[a] == idx) {
-        ++a;
-        continue;
+      const int g = reqs[idx].graph;
+      bool admit = !blocked[g] && !flying[g] && !(lane && lane->busy(g)) &&
+                   (int)(flight.size() + fresh.size()) < slots;
+      blocked[g] = 1;  // later requests of g wait for this one
+      if (admit) {
+        bool granted = false;
+        if ((rc = lease_scratch(m, g, &granted)) != DLP_OK) return rc;
+        if (!granted) ++m->deferrals;
+        admit = granted;
       }
-      rest.push_back(idx);
+      if (admit) fresh.push_back({g, idx, 0});
+      else rest.push_back(idx);
     }
     pending.swap(rest);
+    if (flight.empty() && fresh.empty()) {
+      if (pending.empty() && nextArrival >= numReqs) break;  // only hand-back work left
+      if (lane && lane->inFlight() > 0) {
+        lane->wait(nextAt, now);  // a graph or its lease comes back, or a request arrives
+      } else if (pending.empty()) {
+        std::this_thread::sleep_for(std::chrono::duration<double>(nextAt - t));
+      } else {  // cannot happen: an idle runner admits the first pending batch
+        dlp_set_error_("dlp_multi_serve: no batch admitted with the runner idle");
+        return DLP_ERR_INTERNAL;
+      }
+      continue;
+    }
+    if (cfg->plain) {  // one batch at a time through the single-graph path
+      const Flight f = fresh[0];
+      dlp_context *c = m->ctx[f.graph];
+      if ((rc = dlp_context_run_batch(c, c->nextBatch)) != DLP_OK) return rc;
+      DLP_HIP(hipStreamSynchronize(c->stream));
+      completion[f.req] = now();
+      return_scratch(m, f.graph);
+      ++nSteps;
+      continue;
+    }
+    /* the new batches' pipelines up to their first frontier */
+    for (const Flight &f : fresh) {
+      dlp_context *c = m->ctx[f.graph];
+      const int b = c->nextBatch;
+      int tag = 0;
+      if ((rc = dlp_ctx_counter_guard(c, m->stream)) != DLP_OK) return rc;
+      if ((rc = dlp_ctx_batch_pre(c, b, dlp_ctx_make_batch(c, b), &tag)) != DLP_OK) return rc;
+      flying[f.graph] = 1;
+      flight.push_back(f);
+    }
+    /* one shared launch: resumed and new graphs, each until its slice ends */
+    const int count = (int)flight.size();
+    for (int i = 0; i < count; ++i) {
+      const Flight &f = flight[i];
+      GraphProp gp = graph_prop(m, f.graph);
+      gp.heavy = heavy;
+      gp.startIter = f.k;
+      if (slice > 0)
+        gp.stopIter = (int)std::min<long long>(gp.maxIter, (long long)f.k + slice);
+      gp.sliceTicks = sliceTicks;
+      m->hStep[i] = gp;
+    }
+    DLP_HIP(hipMemcpyAsync(m->dStep, m->hStep, sizeof(GraphProp) * count, hipMemcpyHostToDevice,
+                           m->stream));
+    if ((rc = dlp_launch_propagate_multi(m->dStep, count, m->shape, m->ldsWords, m->stream)) !=
+        DLP_OK)
+      return rc;
+    DLP_HIP(hipMemcpyAsync(m->hExit, m->dExit, sizeof(int) * m->G, hipMemcpyDeviceToHost,
+                           m->stream));
+    DLP_HIP(hipStreamSynchronize(m->stream));
+    ++nSteps;
+    /* finished graphs: batch end; handed back: to a lane; sliced: carried */
+    std::vector<int> done;
+    carried.clear();
+    for (const Flight &f : flight) {
+      const int e = m->hExit[f.graph];
+      const int maxIter = m->ctx[f.graph]->cfg.maxIterations;
+      if (e == -1) {
+        if ((rc = post(f.graph)) != DLP_OK) return rc;
+        done.push_back(f.graph);
+      } else if (e >= 0 && e <= maxIter && heavy > 0) {
+        flying[f.graph] = 0;
+        laneReq[f.graph] = f.req;
+        if ((rc = lane->push(f.graph, e)) != DLP_OK) return rc;
+      } else if (e <= -2 && -2 - e >= f.k && -2 - e < maxIter) {
+        carried.push_back({f.graph, f.req, -2 - e});
+      } else {
+        dlp_set_errorf_("graph %d: propagation launch exit %d from iteration %d", f.graph, e, f.k);
+        return DLP_ERR_INTERNAL;
+      }
+    }
+    if (!done.empty()) {
+      DLP_HIP(hipStreamSynchronize(m->stream));
+      const double tDone = now();
+      for (const Flight &f : flight)
+        if (std::find(done.begin(), done.end(), f.graph) != done.end()) {
+          completion[f.req] = tDone;
+          flying[f.graph] = 0;
+          return_scratch(m, f.graph);
+        }
+    }
+    flight.swap(carried);
   }
   if (lane) {
     lane->drain();

@@ -17,6 +17,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "dlp_kernels.h"
@@ -279,6 +280,72 @@ int trace_pre_propagation(dlp_context *c, const DBatch &bt, int markTag) {
 
 }  // namespace
 
+/* ------------------------------------------------------------------------ */
+/* per-batch scratch                                                        */
+/* ------------------------------------------------------------------------ */
+int64_t dlp_ctx_scratch_layout(dlp_context *c, long long ins, long long e, char *base,
+                               bool bind) {
+  DState &d = c->d;
+  const long long N = std::max(c->N, 1);
+  const long long mi = std::max(ins, 1LL), me = std::max(e, 1LL);
+  const long long scanInts = (long long)std::max(
+      {dlp_scan_scratch_ints(d.numWords), dlp_scan_scratch_ints(me), dlp_scan_scratch_ints(mi),
+       dlp_scan_scratch_ints(N), dlp_radix_scratch_ints(mi)});
+  int64_t off = 0;
+  auto carve = [&](auto **p, long long count) {
+    using Ptr = std::remove_reference_t<decltype(*p)>;
+    if (bind) *p = base ? reinterpret_cast<Ptr>(base + off) : nullptr;
+    off += ((int64_t)count * 4 + 255) / 256 * 256;  // every buffer holds 4-byte items
+  };
+  carve(&d.cand, N);
+  carve(&d.q0, N);
+  carve(&d.q1, N);
+  carve(&d.touched, N);
+  carve(&d.S, mi);
+  carve(&d.subSrc, 2 * me);
+  carve(&d.subDst, 2 * me);
+  carve(&d.subW, 2 * me);
+  for (float **p : {&d.p0, &d.p1, &d.red0, &d.red1, &d.W0, &d.W1}) carve(p, mi);
+  for (int **p : {&d.keyA, &d.valA, &d.keyB, &d.valB, &d.keyC, &d.valC, &d.segStart, &d.segEnd})
+    carve(p, mi);
+  carve(&d.scratch, scanInts);
+  return off;
+}
+
+int64_t dlp_ctx_batch_scratch_bytes(const dlp_context *c, int b) {
+  return dlp_ctx_scratch_layout(const_cast<dlp_context *>(c), c->insOff[b + 1] - c->insOff[b],
+                                c->eOff[b + 1] - c->eOff[b], nullptr, false);
+}
+
+int dlp_ctx_own_scratch(dlp_context *c, bool own) {
+  if (!own) {
+    if (c->scratchAlloc) {
+      DLP_HIP(hipStreamSynchronize(c->stream));
+      DLP_HIP(hipFree(c->scratchAlloc));
+      c->deviceBytes -= c->scratchAllocBytes;
+    }
+    c->scratchAlloc = nullptr;
+    c->scratchAllocBytes = 0;
+    dlp_ctx_scratch_layout(c, c->maxIns, c->maxE, nullptr, true);
+    return DLP_OK;
+  }
+  if (c->scratchAlloc) return DLP_OK;
+  const int64_t bytes = dlp_ctx_scratch_layout(c, c->maxIns, c->maxE, nullptr, false);
+  void *p = nullptr;
+  const hipError_t e = hipMalloc(&p, (size_t)bytes);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    dlp_set_errorf_("hipMalloc of %lld bytes of batch scratch failed (%s)", (long long)bytes,
+                    hipGetErrorString(e));
+    return DLP_ERR_NOMEM;
+  }
+  c->scratchAlloc = p;
+  c->scratchAllocBytes = bytes;
+  c->deviceBytes += bytes;
+  dlp_ctx_scratch_layout(c, c->maxIns, c->maxE, (char *)p, true);
+  return DLP_OK;
+}
+
 extern "C" int dlp_context_create(dlp_context **out, const dlp_config *cfg,
                                   const dlp_graph_view *g, const float *trueLabels,
                                   const dlp_batch_view *bv, void *stream) {
@@ -395,33 +462,9 @@ extern "C" int dlp_context_create(dlp_context **out, const dlp_config *cfg,
   TRY(c->alloc(&d.compId, N));
   TRY(c->alloc(&d.par, N));
   TRY(c->alloc(&d.newBits, d.numWords));
-  TRY(c->alloc(&d.cand, N));
-  TRY(c->alloc(&d.q0, N));
-  TRY(c->alloc(&d.q1, N));
-  TRY(c->alloc(&d.touched, N));
-  const long long mi = std::max(maxIns, 1);
-  TRY(c->alloc(&d.S, mi));
-  TRY(c->alloc(&d.subSrc, 2LL * std::max(maxE, 1)));
-  TRY(c->alloc(&d.subDst, 2LL * std::max(maxE, 1)));
-  TRY(c->alloc(&d.subW, 2LL * std::max(maxE, 1)));
-  TRY(c->alloc(&d.p0, mi));
-  TRY(c->alloc(&d.p1, mi));
-  TRY(c->alloc(&d.keyA, mi));
-  TRY(c->alloc(&d.valA, mi));
-  TRY(c->alloc(&d.keyB, mi));
-  TRY(c->alloc(&d.valB, mi));
-  TRY(c->alloc(&d.keyC, mi));
-  TRY(c->alloc(&d.valC, mi));
-  TRY(c->alloc(&d.segStart, mi));
-  TRY(c->alloc(&d.segEnd, mi));
-  TRY(c->alloc(&d.red0, mi));
-  TRY(c->alloc(&d.red1, mi));
-  TRY(c->alloc(&d.W0, mi));
-  TRY(c->alloc(&d.W1, mi));
-  size_t scratch = std::max({dlp_scan_scratch_ints(d.numWords), dlp_scan_scratch_ints(maxE),
-                             dlp_scan_scratch_ints(mi), dlp_scan_scratch_ints(N),
-                             dlp_radix_scratch_ints(mi)});
-  TRY(c->alloc(&d.scratch, (long long)scratch));
+  c->maxIns = maxIns;
+  c->maxE = maxE;
+  TRY(dlp_ctx_own_scratch(c, true));
   TRY(c->alloc(&d.ctr, 1));
   TRY(c->alloc(&d.records, B));
   c->iterSizesCap = std::max(1, std::min(cfg->maxIterations, 1 << 20));

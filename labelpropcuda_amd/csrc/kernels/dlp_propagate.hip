@@ -1802,23 +1802,30 @@ __device__ __forceinline__ void wg_lane_rows(const PropArgs &a, const float *Fol
 }
 
 /* Iterations k, k+1, ... by one work-group over HBM, while the frontier is
- * non-empty, k < maxIter, not (frontier <= threshold and k >= resumeAt) and
- * not (heavy > 0 and frontier > heavy): the same global state transitions as
- * one k_propagate launch each.  On return S.misc[0] holds the frontier size
- * of iteration k. */
+ * non-empty, k < maxIter, not (frontier <= threshold and k >= resumeAt), not
+ * (heavy > 0 and frontier > heavy) and not past the deadline (wall-clock
+ * ticks, 0 = none; checked once k > kFirst): the same global state
+ * transitions as one k_propagate launch each.  On return S.misc[0] holds
+ * the frontier size of iteration k and S.misc[1] is 1 if the deadline
+ * ended the iterations. */
 template <int LT>
 __device__ __forceinline__ void wg_iterations(const PropArgs &a, int &k, int maxIter, int threshold,
-                              int resumeAt, int heavy, LocalShared &S, LocalProf &prof) {
+                              int resumeAt, int heavy, long long deadline, int kFirst,
+                              LocalShared &S, LocalProf &prof) {
   constexpr int kWgLaneRowMax = 16;
   DevCounters *c = a.ctr;
   const int tid = threadIdx.x, lane = tid & 63;
   const int stampBase = c->stampBase;
   while (true) {
-    __syncthreads();  // every thread has read the previous S.misc[0]
-    if (tid == 0) S.misc[0] = ld_agent(&c->qcount[k % 3]);
+    __syncthreads();  // every thread has read the previous S.misc[0..1]
+    if (tid == 0) {
+      S.misc[0] = ld_agent(&c->qcount[k % 3]);
+      S.misc[1] = (deadline > 0 && k > kFirst && wall_clock64() >= deadline) ? 1 : 0;
+    }
     __syncthreads();
     const int n = S.misc[0];
-    if (n == 0 || k >= maxIter || (n <= threshold && k >= resumeAt) || (heavy > 0 && n > heavy))
+    if (n == 0 || k >= maxIter || (n <= threshold && k >= resumeAt) || (heavy > 0 && n > heavy) ||
+        S.misc[1])
       break;  // uniform
     if (tid == 0) {
       c->qcount[(k + 2) % 3] = 0;
@@ -1926,12 +1933,19 @@ __device__ __forceinline__ void wg_prof_flush(const PropArgs &a, LocalProf &prof
   prof = LocalProf{};
 }
 
-/* One work-group per graph runs the propagation of its graph's batch:
- * LDS-resident passes while the frontier is at most the threshold (and no
- * overflow in the last `backoff` iterations), HBM iterations otherwise.  A
- * frontier above `heavy` (when > 0) ends the graph's work at the start of
- * that iteration, with the state a multi-block launch would leave, and
- * *exitIter tells the host where to resume; otherwise *exitIter = -1. */
+/* One work-group per graph runs the propagation of its graph's batch from
+ * iteration startIter: LDS-resident passes while the frontier is at most the
+ * threshold (and no overflow in the last `backoff` iterations), HBM
+ * iterations otherwise.  It stops at the start of an iteration when
+ *   - the frontier is empty or maxIter is reached: *exitIter = -1;
+ *   - the frontier is above `heavy` (when > 0): *exitIter = k, with the
+ *     state a multi-block launch would leave (resume on the full GPU);
+ *   - k reaches stopIter < maxIter, or the launch has run sliceTicks
+ *     wall-clock ticks and the graph at least one iteration:
+ *     *exitIter = dlp_slice_exit(k) (resume in a later launch).
+ * Every stop leaves the queue, counters and labels of iteration k on the
+ * device, so the iterations and results do not depend on where a graph was
+ * stopped and resumed. */
 template <int LT, int SR>
 __global__ __launch_bounds__(LT) void k_propagate_multi(const GraphProp *graphs) {
   extern __shared__ int4 pool4[];
@@ -1940,32 +1954,44 @@ __global__ __launch_bounds__(LT) void k_propagate_multi(const GraphProp *graphs)
   const PropArgs a = gp.a;
   const LocalArgs l = gp.l;
   const int maxIter = gp.maxIter, threshold = gp.threshold, backoff = gp.backoff;
+  const int kFirst = gp.startIter;
+  const int stopAt = min(gp.stopIter, maxIter);
+  const long long deadline = gp.sliceTicks > 0 ? wall_clock64() + gp.sliceTicks : 0;
   if (threadIdx.x < 64) S.dummyStamp[threadIdx.x] = DLP_INT_MAX;
   LocalProf lprof, gprof;
   const int heavy = gp.heavy;
-  int k = 0, resumeAt = 0, handBack = -1;
+  int k = kFirst, resumeAt = kFirst, exitCode = -1;
+  bool more = false; /* frontier of iteration k non-empty at the stop */
   if (l.ldsWords > 0 && threshold > 0) {
     while (true) {
-      wg_iterations<LT>(a, k, maxIter, threshold, resumeAt, heavy, S, gprof);
+      wg_iterations<LT>(a, k, stopAt, threshold, resumeAt, heavy, deadline, kFirst, S, gprof);
       const int n = S.misc[0];
-      if (n == 0 || k >= maxIter) break;  // uniform
-      if (heavy > 0 && n > heavy) {
-        handBack = k;
+      if (n == 0) break;  // uniform
+      if (k >= stopAt || S.misc[1]) {
+        more = true;
         break;
       }
-      const int reason = local_passes<LT, SR, false>(a, l, k, maxIter, (int *)pool4, S, lprof, -1);
+      if (heavy > 0 && n > heavy) {
+        exitCode = k;
+        break;
+      }
+      const int reason = local_passes<LT, SR, false>(a, l, k, stopAt, (int *)pool4, S, lprof, -1);
+      if (reason == DLP_LOCAL_CAPPED) more = true;
       if (reason != DLP_LOCAL_TOO_BIG) break;
       resumeAt = k + backoff;
       wg_global_sync();
     }
   } else {
-    wg_iterations<LT>(a, k, maxIter, -1, maxIter, heavy, S, gprof);
+    wg_iterations<LT>(a, k, stopAt, -1, stopAt, heavy, deadline, kFirst, S, gprof);
     const int n = S.misc[0];
-    if (n > 0 && k < maxIter && heavy > 0 && n > heavy) handBack = k;
+    if (n > 0 && k < stopAt && !S.misc[1] && heavy > 0 && n > heavy) exitCode = k;
+    else if (n > 0) more = true;
   }
+  /* a non-empty frontier below maxIter that no hand-back took: resume later */
+  if (exitCode < 0 && more && k < maxIter) exitCode = dlp_slice_exit(k);
   local_prof_flush(a, lprof);
   wg_prof_flush(a, gprof);
-  if (threadIdx.x == 0 && gp.exitIter) *gp.exitIter = handBack;
+  if (threadIdx.x == 0 && gp.exitIter) *gp.exitIter = exitCode;
 }
 
 }  // namespace

@@ -567,14 +567,33 @@ class Multi:
         arr = np.ascontiguousarray(graphs, np.int32)
         check(self._lib.dlp_multi_run_next(self._h, _ptr(arr, C.c_int), len(arr)))
 
+    def set_scratch_pool(self, poolBytes: int):
+        """poolBytes > 0: batches draw their scratch from one shared pool of
+        that size; 0: every graph its own scratch (the default)."""
+        check(self._lib.dlp_multi_set_scratch_pool(self._h, int(poolBytes)))
+
+    def pool_stats(self) -> dict:
+        st = _lib.PoolStats()
+        check(self._lib.dlp_multi_pool_stats(self._h, C.byref(st)))
+        return {name: getattr(st, name) for name, _ in st._fields_}
+
+    def batch_scratch_bytes(self, g: int, b: int) -> int:
+        v = C.c_int64()
+        check(self._lib.dlp_multi_batch_scratch_bytes(self._h, g, b, C.byref(v)))
+        return v.value
+
     def serve(self, requests, maxSlots: int = 64, hbmBudgetBytes: int = 0,
-              plain: bool = False, handBackLanes: int = 0):
+              plain: bool = False, handBackLanes: int = 0, sliceIterations: int = 0,
+              sliceMs: float = 0.0):
         """requests: sequence of (graph, arrival seconds), sorted by arrival.
-        Returns (completion seconds per request, steps)."""
+        Returns (completion seconds per request, steps).  sliceIterations
+        -1 with sliceMs < 0 runs every step to completion (step-level
+        admission)."""
         n = len(requests)
         reqs = (_lib.ServeRequest * max(n, 1))(*[_lib.ServeRequest(int(g), float(t))
                                                   for g, t in requests])
-        cfg = _lib.ServeConfig(maxSlots, hbmBudgetBytes, 1 if plain else 0, handBackLanes)
+        cfg = _lib.ServeConfig(maxSlots, hbmBudgetBytes, 1 if plain else 0, handBackLanes,
+                               sliceIterations, sliceMs)
         done = np.zeros(max(n, 1), np.float64)
         steps = C.c_int()
         check(self._lib.dlp_multi_serve(self._h, reqs, n, C.byref(cfg),

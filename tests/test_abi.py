@@ -147,7 +147,7 @@ def test_ctypes_structs_match_the_header_layout(tmp_path):
              "dlp_multi_tuning": _lib.MultiTuning, "dlp_serve_request": _lib.ServeRequest,
              "dlp_serve_config": _lib.ServeConfig, "dlp_prop_profile": _lib.PropProfile,
              "dlp_irlp_tuning": _lib.IrlpTuning, "dlp_irlp_result": _lib.IrlpResult,
-             "dlp_shape_key": _lib.ShapeKey}
+             "dlp_shape_key": _lib.ShapeKey, "dlp_pool_stats": _lib.PoolStats}
     assert set(header_structs()) == set(pairs), "a header struct has no ctypes mirror"
     lines = ['#include <stddef.h>', '#include <stdio.h>', '#include "dynlp/dynlp_c.h"',
              'int main(void) {']

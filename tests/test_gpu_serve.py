@@ -44,19 +44,27 @@ def requests(tenants, rng, spread):
     return sorted(reqs, key=lambda r: r[1])
 
 
-@pytest.mark.parametrize("slots,budget,plain,spread,heavy,lanes", [
-    (64, 0, False, 0.0, 0, 0),        # everything at once: maximal mixing
-    (3, 0, False, 0.05, 0, 0),        # few slots, staggered arrivals
-    (64, 4096, False, 0.0, 0, 0),     # tiny HBM budget: one batch per step
-    (1, 0, True, 0.0, 0, 0),          # plain: one batch at a time, single-graph path
-    (64, 0, False, 0.0, 1, 0),        # every step hands its graphs back (default lanes)
-    (64, 0, False, 0.0, 1, 1),        # ... to a single lane
-    (4, 0, False, 0.02, 200, 0),      # some graphs handed back, staggered
-    (8, 0, False, 0.01, 100, 5),      # five lanes, staggered
-    (64, 0, False, 0.0, -1, 0),       # no hand-back
+# budget "one": a scratch pool that holds the largest batch (smaller ones
+# may share it); slice (iterations, ms): 0/0.0 = the defaults, -1/-1.0 = each
+# step runs its batches to the end
+@pytest.mark.parametrize("slots,budget,plain,spread,heavy,lanes,slice", [
+    (64, 0, False, 0.0, 0, 0, (0, 0.0)),        # everything at once: maximal mixing
+    (3, 0, False, 0.05, 0, 0, (0, 0.0)),        # few slots, staggered arrivals
+    (64, "one", False, 0.0, 0, 0, (0, 0.0)),    # pool for about one batch: deferrals
+    (1, 0, True, 0.0, 0, 0, (0, 0.0)),          # plain: one batch at a time, single-graph path
+    (64, 0, False, 0.0, 1, 0, (0, 0.0)),        # every step hands its graphs back (default lanes)
+    (64, 0, False, 0.0, 1, 1, (0, 0.0)),        # ... to a single lane
+    (4, 0, False, 0.02, 200, 0, (0, 0.0)),      # some graphs handed back, staggered
+    (8, 0, False, 0.01, 100, 5, (0, 0.0)),      # five lanes, staggered
+    (64, 0, False, 0.0, -1, 0, (0, 0.0)),       # no hand-back
+    (64, 0, False, 0.0, -1, 0, (-1, -1.0)),     # step-level admission (no slices)
+    (5, 0, False, 0.01, -1, 0, (7, -1.0)),      # 7-iteration slices: many resumptions
+    (64, 0, False, 0.0, 300, 2, (3, -1.0)),     # 3-iteration slices beside hand-backs
+    (64, 0, False, 0.0, -1, 0, (-1, 0.005)),    # 5-microsecond time slices
+    (6, "one", False, 0.01, 150, 3, (11, -1.0)),  # pool, slices and hand-backs together
 ])
 def test_serving_same_alone_and_in_mix(gpu_available, tenants, slots, budget, plain, spread,
-                                       heavy, lanes):
+                                       heavy, lanes, slice):
     ref = alone(tenants, 3000)
     cfgs = [L.make_config(t, maxIterations=3000) for t in tenants]
     rng = np.random.default_rng(11)
@@ -65,12 +73,24 @@ def test_serving_same_alone_and_in_mix(gpu_available, tenants, slots, budget, pl
         m.reset()
         m.run_next(list(range(len(tenants))))  # seed batches: tenants become resident
         reqs = requests(tenants, rng, spread)
+        if budget == "one":
+            budget = max(m.batch_scratch_bytes(t, b) for t, inp in enumerate(tenants)
+                         for b in range(1, inp.numBatches))
+        print(f"serve slots={slots} budget={budget} plain={plain} heavy={heavy} "
+              f"slice={slice}", flush=True)
         done, steps = m.serve(reqs, maxSlots=slots, hbmBudgetBytes=budget, plain=plain,
-                              handBackLanes=lanes)
+                              handBackLanes=lanes, sliceIterations=slice[0], sliceMs=slice[1])
         m.synchronize()
         assert np.all(done >= np.array([r[1] for r in reqs]))
-        if plain or budget:
+        if plain:
             assert steps == len(reqs)
+        if budget:
+            st = m.pool_stats()
+            assert st["capacity"] == budget // 256 * 256
+            assert st["inUse"] == 0 and 0 < st["peak"] <= st["capacity"]
+            assert st["leases"] == len(reqs)
+            if slots > 1 and spread == 0.0:
+                assert st["deferrals"] > 0  # the batches arrived together
         for t, inp in enumerate(tenants):
             assert np.array_equal(bits(m.labels(t)), bits(ref[t][0])), f"tenant {t}"
             for b in range(inp.numBatches):
