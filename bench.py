@@ -859,7 +859,8 @@ def main() -> int:
     if not args.no_batched:
         out["batched"] = batched_leg(args, progress, autotune=False)
     if not args.no_serving:
-        out["serving"] = serving_leg(args, progress, [])
+        # saturated, and Poisson arrivals at twice the plain path's capacity
+        out["serving"] = serving_leg(args, progress, [2.0])
     if args.calibrate_copy_mib > 0:
         out["copy_calibration"] = copy_calibration(args.calibrate_copy_mib)
     print(json.dumps(out))

@@ -371,6 +371,17 @@ int dlp_counter_guard_plan(int64_t stampBound, int64_t genBound, int maxIteratio
  * values at a batch boundary (tests of the guards start near the limits);
  * values may only grow. */
 int dlp_context_set_counter_bases(dlp_context *ctx, int stampBase, int localGen);
+
+/* How a batch's CSR update chooses rows (the reference rebuilds the whole
+ * host CSR every batch, dynLP.cu:111-167): mode 0 = only the rows the
+ * batch can change -- those naming a vertex of its lists or of the
+ * previous batch's deletions -- when the full graph is symmetric and they
+ * are at most a quarter of the rows, else every row (the default); 1 =
+ * every row; 2 = only the rows the batch can change whenever the full
+ * graph is symmetric.  mode < 0 keeps the mode.  *symmetric (nullable)
+ * receives 1 if the full graph names v in u's row exactly as often as u in
+ * v's row.  The CSR, and everything after it, is the same in every mode. */
+int dlp_context_row_update(dlp_context *ctx, int mode, int *symmetric);
 /* Current device values of both counters (synchronises). */
 int dlp_context_counter_bases(dlp_context *ctx, int *stampBase, int *localGen);
 

@@ -22,6 +22,12 @@ struct dlp_context {
   float tau = 0.5f;
   int N = 0, B = 0;
   long long nnzFull = 0;
+  /* the full graph names v in u's row exactly as often as u in v's row */
+  bool symmetric = false;
+  /* per batch: rows the batch's lists (and the previous batch's deletions)
+   * can change, with their full-graph neighbours, counted with repeats */
+  std::vector<int64_t> rowsBound;
+  int rowUpdate = 0; /* dlp_context_row_update mode */
   std::vector<int64_t> insOff, gtOff, delOff, eOff;
   std::vector<void *> allocs;
   int64_t deviceBytes = 0;

@@ -70,6 +70,12 @@ struct DevCounters {
   unsigned long long profLocalBuilds;       /* LDS working sets built         */
   unsigned long long profLocalWaveUpdates;  /* of profLocalOwnerUpdates: in the
                                                single-wave owner kernel      */
+  int rowCount;        /* rows listed for the batch's CSR rebuild           */
+  int pad2;
+  /* accuracy numerator / denominator over all vertices, kept up to date
+   * from the vertices each batch changes (zeroed by a reset only) */
+  unsigned long long accCorrect;
+  unsigned long long accTotal;
 };
 
 /* exit reasons of the LDS-resident propagation kernel */

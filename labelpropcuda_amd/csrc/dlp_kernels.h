@@ -39,6 +39,10 @@ struct PropArgs {
    * than blockRowMin (0 = never) by a whole work-group */
   int waveRowMin;
   int blockRowMin;
+  /* 1: a row names v exactly when v's row names it (the pipeline's active
+   * CSR over a symmetric full graph); the multi-wave register-mask kernels
+   * read the next frontier off the rows and need it */
+  int symmetric;
 };
 constexpr int kDefaultWaveRowMin = 64;
 constexpr int kDefaultBlockRowMin = 1024;
@@ -242,6 +246,9 @@ struct DState {
   float *lab0, *lab1;
   const float *trueLab;
   int *iterStamp, *markStamp, *compId, *par;
+  int *rowStamp;  /* batch tag of the last CSR rebuild that listed the row  */
+  int *accState;  /* accuracy contribution: bit 0 counted (host status
+                     ACTIVE), bit 1 predicted class == true class          */
   uint32_t *newBits;
   int numWords;
   /* per-batch scratch */
@@ -323,7 +330,13 @@ int dlp_restart_local_gens(const LocalArgs &l, DevCounters *ctr, hipStream_t s);
 
 int dlp_reset_state(const DState &d, hipStream_t s);
 int dlp_batch_begin(const DState &d, const DBatch &bt, hipStream_t s);
-int dlp_csr_update(const DState &d, const DBatch &bt, int markTag, hipStream_t s);
+/* The active CSR of batch bt.  rowBound < 0: every row from the full graph
+ * (one pass over it); else only the rows naming a vertex of the batch's
+ * lists or of the previous batch's deletion list (at most rowBound rows):
+ * valid when the full graph is symmetric, so that those are the listed
+ * vertices' own rows and their neighbours' rows. */
+int dlp_csr_update(const DState &d, const DBatch &bt, int markTag, long long rowBound,
+                   hipStream_t s);
 int dlp_sorted_inserted(const DState &d, hipStream_t s);
 int dlp_sparsify(const DState &d, const DBatch &bt, float tau, int markTag, hipStream_t s);
 int dlp_components(const DState &d, const DBatch &bt, hipStream_t s);

@@ -103,8 +103,8 @@ int dlp_run_propagation(const PropArgs &a, const LocalArgs *l, int maxIter,
         level = maskMin;
         while (level < kMaskLevels && 2 * n > kMaskLevelThreads[level]) ++level;
         /* the multi-wave mask kernels read the next frontier off symmetric
-         * rows: not for a caller's CSR (status array given) */
-        if (a.status && level > 0) level = kMaskLevels;
+         * rows: not for an asymmetric CSR */
+        if (!a.symmetric && level > 0) level = kMaskLevels;
         if (level < kMaskLevels) la.shape = kLocalMaskShape + level;
       }
       if ((rc = begin_timed(timing, s, true, 1)) != DLP_OK) return rc;

@@ -17,6 +17,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <type_traits>
 #include <vector>
 
@@ -72,6 +73,49 @@ static int need_device() {
 #include "dlp_context.h"
 
 namespace {
+
+/* splitmix64 finaliser: a well-mixed 64-bit hash of an ordered vertex pair */
+uint64_t pair_hash(uint32_t u, uint32_t v) {
+  uint64_t x = ((uint64_t)u << 32 | v) + 0x9e3779b97f4a7c15ull;
+  x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ull;
+  x = (x ^ (x >> 27)) * 0x94d049bb133111ebull;
+  return x ^ (x >> 31);
+}
+
+/* Whether the full graph's entries, (row, column) pairs with column >= 0,
+ * form the same multiset as their reverses: then a row names v exactly
+ * when v's row names it.  Compared through sums (mod 2^64) of a hash of
+ * each pair and of its reverse -- equal for every symmetric graph, and
+ * equal for an asymmetric one only by a hash collision (probability about
+ * 2^-64).  Threads over row ranges. */
+bool csr_symmetric(const dlp_graph_view *g) {
+  const int N = g->numNodes;
+  if (N <= 0 || g->numEdges <= 0) return true;
+  const int T = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  std::vector<uint64_t> fwd(T, 0), rev(T, 0);
+  std::vector<std::thread> th;
+  for (int t = 0; t < T; ++t)
+    th.emplace_back([&, t] {
+      const int lo = (int)((long long)N * t / T), hi = (int)((long long)N * (t + 1) / T);
+      uint64_t f = 0, r = 0;
+      for (int u = lo; u < hi; ++u)
+        for (int e = g->rowPtr[u]; e < g->rowPtr[u + 1]; ++e) {
+          const int v = g->colInd[e];
+          if (v < 0) continue;
+          f += pair_hash((uint32_t)u, (uint32_t)v);
+          r += pair_hash((uint32_t)v, (uint32_t)u);
+        }
+      fwd[t] = f;
+      rev[t] = r;
+    });
+  for (std::thread &x : th) x.join();
+  uint64_t f = 0, r = 0;
+  for (int t = 0; t < T; ++t) {
+    f += fwd[t];
+    r += rev[t];
+  }
+  return f == r;
+}
 
 /* host-side validation of the inputs (the reference trusts them; bad ids
  * would make device kernels index out of bounds) */
@@ -383,6 +427,7 @@ extern "C" int dlp_context_create(dlp_context **out, const dlp_config *cfg,
     tau = (g->numEdges == 0 || count == 0) ? 0.5f : (float)(sum / (double)count);
   }
   c->tau = tau;
+  c->symmetric = csr_symmetric(g);
 
   /* batch offsets (host) and concatenated device lists */
   auto offs = [&](const int64_t *o, std::vector<int64_t> &dst) {
@@ -397,6 +442,22 @@ extern "C" int dlp_context_create(dlp_context **out, const dlp_config *cfg,
   for (int b = 0; b < B; ++b) {
     maxIns = std::max<int>(maxIns, (int)(c->insOff[b + 1] - c->insOff[b]));
     maxE = std::max<int>(maxE, (int)(c->eOff[b + 1] - c->eOff[b]));
+  }
+  /* rows each batch can change (the incremental CSR update's bound) */
+  c->rowsBound.assign(B, 0);
+  for (int b = 0; b < B; ++b) {
+    auto rows = [&](const int64_t *o, const int *ids, int bb) {
+      int64_t r = 0;
+      for (int64_t i = o[bb]; i < o[bb + 1]; ++i) {
+        const int x = ids[i];
+        r += 1 + (x >= 0 && x < N ? g->rowPtr[x + 1] - g->rowPtr[x] : 0);
+      }
+      return r;
+    };
+    c->rowsBound[b] = rows(bv->insertedOffsets, bv->insertedNodeIds, b) +
+                      rows(bv->deletedOffsets, bv->deletedNodeIds, b) +
+                      rows(bv->groundTruthOffsets, bv->groundTruthNodeIds, b) +
+                      (b > 0 ? rows(bv->deletedOffsets, bv->deletedNodeIds, b - 1) : 0);
   }
   std::vector<int> batchMap(N, -1);
   for (int b = 0; b < B; ++b)
@@ -461,6 +522,8 @@ extern "C" int dlp_context_create(dlp_context **out, const dlp_config *cfg,
   TRY(c->alloc(&d.markStamp, N));
   TRY(c->alloc(&d.compId, N));
   TRY(c->alloc(&d.par, N));
+  TRY(c->alloc(&d.rowStamp, N));
+  TRY(c->alloc(&d.accState, N));
   TRY(c->alloc(&d.newBits, d.numWords));
   c->maxIns = maxIns;
   c->maxE = maxE;
@@ -757,6 +820,16 @@ int dlp_ctx_counter_bases(dlp_context *c, int *stampBase, int *localGen) {
   return DLP_OK;
 }
 
+extern "C" int dlp_context_row_update(dlp_context *c, int mode, int *symmetric) {
+  if (!c || mode > 2) {
+    dlp_set_error_("dlp_context_row_update: NULL context or mode not in {0, 1, 2}");
+    return DLP_ERR_INVALID;
+  }
+  if (mode >= 0) c->rowUpdate = mode;
+  if (symmetric) *symmetric = c->symmetric ? 1 : 0;
+  return DLP_OK;
+}
+
 extern "C" int dlp_context_set_counter_bases(dlp_context *c, int stampBase, int localGen) {
   if (!c) {
     dlp_set_error_("dlp_context_set_counter_bases: NULL context");
@@ -779,7 +852,12 @@ int dlp_ctx_batch_pre(dlp_context *c, int b, const DBatch &bt, int *markTag) {
   int rc;
   if ((rc = dlp_batch_begin(d, bt, s)) != DLP_OK) return rc;
   *markTag = ++c->markSeq;
-  if ((rc = dlp_csr_update(d, bt, *markTag, s)) != DLP_OK) return rc;
+  /* only the rows the batch can change, when they are few and the full
+   * graph is symmetric (rows naming a vertex = rows of its neighbours) */
+  const bool listed = c->symmetric && c->rowUpdate != 1 &&
+                      (c->rowUpdate == 2 || c->rowsBound[b] <= (int64_t)c->N / 4);
+  const long long rowBound = listed ? c->rowsBound[b] : -1;
+  if ((rc = dlp_csr_update(d, bt, *markTag, rowBound, s)) != DLP_OK) return rc;
   if (bt.nIns > 0 && bt.nE > 0) {
     if (c->lastCCBatch >= 0) {
       DBatch prev = dlp_ctx_make_batch(c, c->lastCCBatch);
@@ -826,6 +904,10 @@ PropArgs dlp_ctx_prop_args(const dlp_context *c) {
   a.delta = c->cfg.delta;
   a.waveRowMin = c->tuning.waveRowMin;
   a.blockRowMin = c->tuning.blockRowMin > 0 ? c->tuning.blockRowMin : 0;
+  /* the active CSR keeps exactly the entries whose two ends are present
+   * (holes for this batch's deletions on both sides), so it is symmetric
+   * when the full graph is */
+  a.symmetric = c->symmetric ? 1 : 0;
   return a;
 }
 

@@ -119,6 +119,8 @@ __global__ void k_reset(DState d) {
     d.compId[v] = -1;
     d.aLen[v] = 0;
     d.aRow[v] = make_int2(d.fRow[v], 0);
+    d.rowStamp[v] = 0;
+    d.accState[v] = 0;
   }
   for (long long w = i; w < d.numWords; w += stride) d.newBits[w] = 0u;
   if (i == 0) {
@@ -174,6 +176,7 @@ __global__ void k_clear_prev(DState d, DBatch bt) {
     c->correct = 0;
     c->total = 0;
     c->coreCount = 0;
+    c->rowCount = 0;
   }
   const int n1 = bt.nPrevIns, n2 = n1 + bt.nPrevDel, n3 = n2 + bt.nPrevGt;
   if (i >= n3) return;
@@ -241,6 +244,45 @@ __global__ void k_finalize(DState d, DBatch bt) {
 /* ------------------------------------------------------------------------ */
 /* pipeline: CSR update with ballot compaction (8 lanes per row)            */
 /* ------------------------------------------------------------------------ */
+/* Row u of the active CSR (dynLP.cu:134-166) rebuilt from its full row by a
+ * group of G = 8 lanes: entries naming a vertex that is present (host
+ * status not DELETED, arrived) are compacted in order into the row's fixed
+ * slot; entries naming a vertex deleted in this batch, and every entry of
+ * the row of such a vertex, become -1 holes (cleanIncomingEdges,
+ * sparsifySubgraph.cu:58-111), and the neighbours named by such a row are
+ * marked (markDeletions).  All 64 lanes of the wave call it; act and u are
+ * group-uniform. */
+__device__ __forceinline__ void csr_rebuild_row(const DState &d, int markTag, bool act, int u,
+                                                int gl, int grp) {
+  constexpr int G = 8;
+  const uint8_t fu = act ? d.vflags[u] : 0;
+  const bool q1 = (fu & VF_DDEL) != 0;  // row of a vertex deleted in this batch
+  const int fb = act ? d.fRow[u] : 0;
+  const int flen = (act && (fu & VF_HACT)) ? d.fRow[u + 1] - fb : 0;
+  int written = 0;
+  for (int c0 = 0; __any(c0 < flen); c0 += G) {
+    const int p = c0 + gl;
+    const bool valid = p < flen;
+    const int v = valid ? d.fCol[fb + p] : -1;
+    const uint8_t fv = (v >= 0) ? d.vflags[v] : 0;
+    const bool keep = valid && v >= 0 && (fv & VF_HACT);
+    const unsigned long long m = dlp_ballot(keep);
+    const unsigned gm = (unsigned)(m >> (grp * G)) & ((1u << G) - 1u);
+    if (keep) {
+      const int pos = fb + written + __popc(gm & ((1u << gl) - 1u));
+      d.aCol[pos] = (q1 || (fv & VF_DDEL)) ? -1 : v;
+      d.aVal[pos] = d.fVal[fb + p];
+    }
+    dlp_mark_candidate(d.markStamp, markTag, d.cand, &d.ctr->numCand, v, keep && q1);
+    written += __popc(gm);
+  }
+  if (act && gl == 0) {
+    d.aLen[u] = written;
+    d.aRow[u] = make_int2(fb, written);
+  }
+}
+
+/* every row of the graph, 8 rows per wave */
 __global__ __launch_bounds__(kT) void k_csr_update(DState d, int markTag) {
   constexpr int G = 8;
   const int lane = threadIdx.x & 63, gl = lane & (G - 1), grp = lane / G;
@@ -249,32 +291,57 @@ __global__ __launch_bounds__(kT) void k_csr_update(DState d, int markTag) {
   for (long long rb = gwave * (64 / G); rb < d.N; rb += nwaves * (64 / G)) {  // wave-uniform
     const long long uu = rb + grp;
     const bool act = uu < d.N;
-    const int u = act ? (int)uu : 0;
-    const uint8_t fu = act ? d.vflags[u] : 0;
-    const bool q1 = (fu & VF_DDEL) != 0;  // row of a vertex deleted in this batch
-    const int fb = act ? d.fRow[u] : 0;
-    const int flen = (act && (fu & VF_HACT)) ? d.fRow[u + 1] - fb : 0;
-    int written = 0;
-    for (int c0 = 0; __any(c0 < flen); c0 += G) {
+    csr_rebuild_row(d, markTag, act, act ? (int)uu : 0, gl, grp);
+  }
+}
+
+/* The rows a batch changes.  A row's content depends on whether its vertex
+ * and the vertices it names are present and whether they are deleted in
+ * this batch; those change only for the vertices of the batch's lists
+ * (deleted, inserted, ground truth) and of the previous batch's deletion
+ * list.  On a symmetric full graph the rows naming such a vertex x are the
+ * rows of x's full-graph neighbours, so the rows to rebuild are the listed
+ * vertices' own rows and their neighbours' rows.  8 lanes per listed
+ * vertex; each row once per batch (rowStamp = the batch's tag) into
+ * rows[0 .. ctr->rowCount). */
+__global__ __launch_bounds__(kT) void k_csr_rows(DState d, DBatch bt, int tag, int *rows) {
+  constexpr int G = 8;
+  const int lane = threadIdx.x & 63, gl = lane & (G - 1), grp = lane / G;
+  const long long n1 = bt.nPrevDel, n2 = n1 + bt.nDel, n3 = n2 + bt.nIns, n4 = n3 + bt.nGt;
+  const long long gwave = ((long long)blockIdx.x * kT + threadIdx.x) >> 6;
+  const long long nwaves = ((long long)gridDim.x * kT) >> 6;
+  for (long long xb = gwave * (64 / G); xb < n4; xb += nwaves * (64 / G)) {  // wave-uniform
+    const long long i = xb + grp;
+    const bool act = i < n4;
+    int x = 0;
+    if (act)
+      x = i < n1 ? bt.prevDel[i]
+                 : (i < n2 ? bt.del[i - n1] : (i < n3 ? bt.ins[i - n2] : bt.gtIds[i - n3]));
+    const int fb = act ? d.fRow[x] : 0;
+    const int flen = act ? d.fRow[x + 1] - fb : 0;
+    /* position -1 is the vertex itself, 0 .. flen-1 its full row */
+    for (int c0 = -1; __any(act && c0 < flen); c0 += G) {
       const int p = c0 + gl;
-      const bool valid = p < flen;
-      const int v = valid ? d.fCol[fb + p] : -1;
-      const uint8_t fv = (v >= 0) ? d.vflags[v] : 0;
-      const bool keep = valid && v >= 0 && (fv & VF_HACT);
-      const unsigned long long m = dlp_ballot(keep);
-      const unsigned gm = (unsigned)(m >> (grp * G)) & ((1u << G) - 1u);
-      if (keep) {
-        const int pos = fb + written + __popc(gm & ((1u << gl) - 1u));
-        d.aCol[pos] = (q1 || (fv & VF_DDEL)) ? -1 : v;
-        d.aVal[pos] = d.fVal[fb + p];
-      }
-      dlp_mark_candidate(d.markStamp, markTag, d.cand, &d.ctr->numCand, v, keep && q1);
-      written += __popc(gm);
+      int v = -1;
+      if (act && p < flen) v = p < 0 ? x : d.fCol[fb + p];
+      const bool fresh = v >= 0 && atomicMax(&d.rowStamp[v], tag) < tag;
+      const int slot = dlp_wave_append(&d.ctr->rowCount, fresh);
+      if (slot >= 0) rows[slot] = v;
     }
-    if (act && gl == 0) {
-      d.aLen[u] = written;
-      d.aRow[u] = make_int2(fb, written);
-    }
+  }
+}
+
+/* the listed rows rows[0 .. ctr->rowCount), 8 rows per wave */
+__global__ __launch_bounds__(kT) void k_csr_update_rows(DState d, int markTag, const int *rows) {
+  constexpr int G = 8;
+  const int lane = threadIdx.x & 63, gl = lane & (G - 1), grp = lane / G;
+  const long long n = d.ctr->rowCount;
+  const long long gwave = ((long long)blockIdx.x * kT + threadIdx.x) >> 6;
+  const long long nwaves = ((long long)gridDim.x * kT) >> 6;
+  for (long long rb = gwave * (64 / G); rb < n; rb += nwaves * (64 / G)) {  // wave-uniform
+    const long long i = rb + grp;
+    const bool act = i < n;
+    csr_rebuild_row(d, markTag, act, act ? rows[i] : 0, gl, grp);
   }
 }
 
@@ -626,24 +693,37 @@ __global__ void k_enforce_gt(DState d, DBatch bt) {
   }
 }
 
-/* computeAccuracy (dynLP.cu:65-85) over vertices whose host status is ACTIVE */
-__global__ __launch_bounds__(kT) void k_accuracy(DState d) {
+/* computeAccuracy (dynLP.cu:65-85): over the vertices whose host status is
+ * ACTIVE, the share whose label (>= 0.5) gives the true class.  Kept as
+ * running totals over all vertices: a vertex's contribution (accState)
+ * changes only when its host status or its label does, i.e. for the
+ * vertices of the batch's lists and those propagation wrote (the touched
+ * list), so only they are visited.  atomicExch makes a vertex listed twice
+ * count its change once. */
+__global__ __launch_bounds__(kT) void k_accuracy_delta(DState d, DBatch bt) {
   __shared__ int lds[4];
-  int corr = 0, tot = 0;
-  for (long long v = (long long)blockIdx.x * kT + threadIdx.x; v < d.N;
-       v += (long long)gridDim.x * kT) {
+  DevCounters *c = d.ctr;
+  const long long n1 = bt.nIns, n2 = n1 + bt.nDel, n3 = n2 + bt.nGt, n4 = n3 + c->touched;
+  int dCorr = 0, dTot = 0;
+  for (long long i = (long long)blockIdx.x * kT + threadIdx.x; i < n4;
+       i += (long long)gridDim.x * kT) {
+    const int v = i < n1 ? bt.ins[i]
+                         : (i < n2 ? bt.del[i - n1] : (i < n3 ? bt.gtIds[i - n2] : d.touched[i - n3]));
+    int st = 0;
     if (d.vflags[v] & VF_HSACT) {
-      int pred = d.lab0[v] >= 0.5f ? 1 : 0;
-      int truth = d.trueLab[v] >= 0.5f ? 1 : 0;
-      corr += (pred == truth);
-      tot += 1;
+      const int pred = d.lab0[v] >= 0.5f ? 1 : 0;
+      const int truth = d.trueLab[v] >= 0.5f ? 1 : 0;
+      st = 1 | (pred == truth ? 2 : 0);
     }
+    const int old = atomicExch(&d.accState[v], st);
+    dCorr += (st == 3) - (old == 3);
+    dTot += (st & 1) - (old & 1);
   }
-  corr = dlp::block_sum(corr, lds);
-  tot = dlp::block_sum(tot, lds);
-  if (threadIdx.x == 0) {
-    atomicAdd(&d.ctr->correct, (unsigned long long)corr);
-    atomicAdd(&d.ctr->total, (unsigned long long)tot);
+  dCorr = dlp::block_sum(dCorr, lds);
+  dTot = dlp::block_sum(dTot, lds);
+  if (threadIdx.x == 0) {  // two's complement: a negative change adds as its wrap-around
+    if (dCorr) atomicAdd(&c->accCorrect, (unsigned long long)(long long)dCorr);
+    if (dTot) atomicAdd(&c->accTotal, (unsigned long long)(long long)dTot);
   }
 }
 
@@ -657,8 +737,10 @@ __global__ void k_batch_record(DState d, int b) {
   r.iterations = c->iterUsed;
   r.touched = c->touched;
   r.updates = c->updates;
-  r.correct = c->correct;
-  r.total = c->total;
+  c->correct = c->accCorrect;
+  c->total = c->accTotal;
+  r.correct = c->accCorrect;
+  r.total = c->accTotal;
   d.records[b] = r;
   c->stampBase += c->iterUsed + 1;
 }
@@ -807,9 +889,22 @@ int dlp_batch_begin(const DState &d, const DBatch &bt, hipStream_t s) {
   return DLP_OK;
 }
 
-int dlp_csr_update(const DState &d, const DBatch &bt, int markTag, hipStream_t s) {
-  (void)bt;
-  /* 8 rows per wave, 32 rows per block */
+int dlp_csr_update(const DState &d, const DBatch &bt, int markTag, long long rowBound,
+                   hipStream_t s) {
+  /* 8 rows (listed vertices) per wave, 32 per block; the row list lives in
+   * the odd-iteration queue, which propagation first writes after this */
+  if (rowBound >= 0) {
+    const long long listed = (long long)bt.nPrevDel + bt.nDel + bt.nIns + bt.nGt;
+    if (listed == 0) return DLP_OK;  // no row changes
+    hipLaunchKernelGGL(k_csr_rows, dim3(dlp_blocks(listed, 32, 65536)), dim3(kT), 0, s, d, bt,
+                       markTag, d.q1);
+    DLP_LAUNCHED("k_csr_rows");
+    hipLaunchKernelGGL(k_csr_update_rows,
+                       dim3(dlp_blocks(std::min<long long>(rowBound, d.N), 32, 65536)), dim3(kT),
+                       0, s, d, markTag, (const int *)d.q1);
+    DLP_LAUNCHED("k_csr_update_rows");
+    return DLP_OK;
+  }
   hipLaunchKernelGGL(k_csr_update, dim3(dlp_blocks(d.N, 32)), dim3(kT), 0, s, d, markTag);
   DLP_LAUNCHED("k_csr_update");
   return DLP_OK;
@@ -906,8 +1001,8 @@ int dlp_batch_end(const DState &d, const DBatch &bt, hipStream_t s) {
     hipLaunchKernelGGL(k_enforce_gt, dim3(dlp_blocks(bt.nGt, kT)), dim3(kT), 0, s, d, bt);
     DLP_LAUNCHED("k_enforce_gt");
   }
-  hipLaunchKernelGGL(k_accuracy, dim3(dlp_blocks(d.N, kT, 2048)), dim3(kT), 0, s, d);
-  DLP_LAUNCHED("k_accuracy");
+  hipLaunchKernelGGL(k_accuracy_delta, dim3(dlp_blocks(d.N, kT, 2048)), dim3(kT), 0, s, d, bt);
+  DLP_LAUNCHED("k_accuracy_delta");
   hipLaunchKernelGGL(k_batch_record, dim3(1), dim3(1), 0, s, d, bt.b);
   DLP_LAUNCHED("k_batch_record");
   return DLP_OK;

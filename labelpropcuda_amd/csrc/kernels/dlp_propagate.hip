@@ -893,11 +893,10 @@ __device__ __forceinline__ float mask_row_label_any(const int (&src)[L], const f
  * records, both labels, and for each of them the core vertices its row
  * names, as V 64-bit masks.  The frontier of an iteration is V 64-bit masks
  * (uniform).  The next one is the changed vertices plus every core vertex
- * some changed vertex's row names: on the pipeline's CSR, which is
- * symmetric (a row names v iff v's row names it), that is a vertex whose
- * own row mask meets the changed mask -- register operations only; on a
- * caller's CSR (status array given) the changed vertices' row masks are
- * OR-reduced over the wave.  No flag stores, no atomics, no barrier between
+ * some changed vertex's row names: on a symmetric CSR (PropArgs::symmetric:
+ * a row names v iff v's row names it) that is a vertex whose own row mask
+ * meets the changed mask -- register operations only; otherwise the
+ * changed vertices' row masks are OR-reduced over the wave.  No flag stores, no atomics, no barrier between
  * waves: an iteration is a short chain of register and LDS operations.
  * New labels go to the LDS label array, where neighbours read them, and to
  * the registers.  Iterations, arithmetic and the state left behind (labels,
@@ -917,7 +916,7 @@ __device__ __forceinline__ int wave_owner_iterations(
   DevCounters *c = a.ctr;
   const int lane = threadIdx.x;  // the work-group is one wave
   int *dumOut = S.dummyOut + lane;
-  const bool symmetric = a.status == nullptr;  // the pipeline's active CSR
+  const bool symmetric = a.symmetric != 0;
   /* frontier k: the queue's core indices as flags in the other queue
    * array, then as masks */
   const int nq = sh[SH_QC0 + k % 3];
@@ -1079,8 +1078,8 @@ __device__ __forceinline__ int wave_owner_iterations(
  * the LDS label array) and store them; every wave publishes its changed
  * mask, and whether a change reached an eligible vertex outside the core,
  * in LDS; one barrier; then each thread's next flag is `changed`, or its row
- * mask meeting some wave's changed mask (the pipeline's CSR is symmetric;
- * a caller's CSR -- status array given -- never comes here).  The size of
+ * mask meeting some wave's changed mask (the CSR is symmetric: an
+ * asymmetric one -- PropArgs::symmetric 0 -- never comes here).  The size of
  * the next frontier is published with the masks of the following
  * iteration, so one barrier per iteration suffices: an iteration whose
  * frontier turns out empty has run with no flagged thread and changed
@@ -1477,10 +1476,10 @@ __device__ __forceinline__ int local_passes(const PropArgs &a, const LocalArgs &
     if constexpr (OWNER && MASK) {
       /* the register-mask owner kernels: one core vertex per thread (cores
        * of at most LT vertices: with more per thread the serial work
-       * outweighs the barriers saved), rows of at most SR entries, a CSR
-       * of the pipeline (symmetric); anything else goes back to the host
+       * outweighs the barriers saved), rows of at most SR entries, a
+       * symmetric CSR; anything else goes back to the host
        * for a larger mask kernel or the flag-based owner kernel */
-      if (!ownerOK || nCore > LT || hasLong || Elong > 0 || (LT > 64 && a.status)) {
+      if (!ownerOK || nCore > LT || hasLong || Elong > 0 || (LT > 64 && !a.symmetric)) {
         reason = DLP_LOCAL_WIDE;
       } else {
         if (tid == 0) sh[SH_MAXLEN] = 0;

@@ -412,6 +412,14 @@ class Context:
         check(self._lib.dlp_context_prop_profile(self._h, C.byref(p)))
         return {name: getattr(p, name) for name, _ in p._fields_}
 
+    def row_update(self, mode: int = -1) -> bool:
+        """Sets the CSR update's row choice (0 by batch size, 1 every row, 2
+        only the rows the batch can change on a symmetric graph; -1 keeps
+        it); returns whether the full graph is symmetric."""
+        sym = C.c_int()
+        check(self._lib.dlp_context_row_update(self._h, mode, C.byref(sym)))
+        return bool(sym.value)
+
     def reset(self):
         check(self._lib.dlp_context_reset(self._h))
 
