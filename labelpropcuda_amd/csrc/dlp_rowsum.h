@@ -184,6 +184,26 @@ __device__ __forceinline__ void lane_row_sums(int len, const Pos &pos, float &W,
   S = ss[LOG];
 }
 
+/* lane_row_sums for one quantity: the same tree over val(j), j < len (zero
+ * beyond).  Used where the weight sum W of a row is known beforehand: it
+ * is the same tree over the same weights every iteration. */
+template <int L, class Val>
+__device__ __forceinline__ float lane_row_sum(int len, const Val &val) {
+  static_assert(L >= 1 && L <= 64 && (L & (L - 1)) == 0, "L must be a power of two <= 64");
+  constexpr int LOG = log2_exact(L);
+  float sv[LOG + 1];
+#pragma unroll
+  for (int i = 0; i < L; ++i) {
+    const int j = bit_reverse(i, LOG);
+    float v = j < len ? val(j) : 0.0f;
+    int b = 0;
+#pragma unroll
+    for (; (i >> b) & 1; ++b) v = sv[b] + v;
+    sv[b] = v;
+  }
+  return sv[LOG];
+}
+
 /* the reference's label update from the row sums (iterativePropagation.cu:
  * 153-171): F = S / W if W > 1e-10f else F_old, clamped to [0,1] */
 __device__ __forceinline__ float new_label(float W, float S, float oldLabel) {

@@ -568,7 +568,8 @@ __device__ __forceinline__ void owner_rows(const PropArgs &a, const LocalArgs &l
                                            float oldLabel, float &nl, bool &changed,
                                            LocalProf &prof) {
   /* all record loads, then all label loads, then the sums: two LDS round
-   * trips for the row */
+   * trips for the row (a weight sum taken once per pass, as the mask
+   * kernels do, costs this kernel more spilled registers than it saves) */
   int tt[L];
   float ww[L], ff[L];
 #pragma unroll
@@ -730,8 +731,8 @@ __device__ __forceinline__ int owner_iterations(
         const int2 *rowS = recS + (len[v] <= SR ? ro[v] : 0);
         if (!__any(lanePath && len[v] > 4))
           owner_rows<(SR > 4 ? 4 : SR), false>(a, l, it, rowS, nullSlot, Fo, Fn, nxt, dumOut,
-                                               lanePath, ci, len[v], hasOut[v], oldLabel, nl,
-                                               changed, prof);
+                                               lanePath, ci, len[v], hasOut[v], oldLabel,
+                                               nl, changed, prof);
         else if (!__any(lanePath && len[v] > SR))
           owner_rows<SR, false>(a, l, it, rowS, nullSlot, Fo, Fn, nxt, dumOut, lanePath, ci,
                                 len[v], hasOut[v], oldLabel, nl, changed, prof);
@@ -853,37 +854,40 @@ __device__ __forceinline__ unsigned long long wave_or64(unsigned long long m) {
 }
 
 /* New label of a register-held row (its first LL <= L records used; rows
- * are padded with null records): the neighbour labels come from the LDS
- * label array Fo.  Lanes without a flagged vertex read the null slot (one
- * address: a broadcast, no bank traffic) and their result is not used. */
+ * are padded with null records of weight 0) whose weight sum W was taken
+ * once per pass (the same tree over the same weights): the neighbour labels
+ * come from the LDS label array Fo.  Lanes without a flagged vertex read
+ * the null slot (one address: a broadcast, no bank traffic) and their
+ * result is not used. */
 template <int LL, int L>
 __device__ __forceinline__ float mask_row_label(const int (&src)[L], const float (&w)[L],
-                                                const float *Fo, int nullSlot, bool f,
+                                                float W, const float *Fo, int nullSlot, bool f,
                                                 float old) {
   float ff[LL];
 #pragma unroll
   for (int p = 0; p < LL; ++p) ff[p] = Fo[f ? src[p] : nullSlot];
-  float W, Sm;
-  dlp::lane_row_sums<LL>(
-      LL,
-      [&](int p, float &ww, float &ss) {
-        ww = f ? w[p] : 0.0f;
-        ss = ww * ff[p];
-      },
-      W, Sm);
+  const float Sm = dlp::lane_row_sum<LL>(LL, [&](int p) { return w[p] * ff[p]; });
   return dlp::new_label(W, Sm, old);
+}
+
+/* the weight sum of a register-held row of at most L records, in the
+ * reference's order */
+template <int L>
+__device__ __forceinline__ float mask_row_weight(const float (&w)[L]) {
+  return dlp::lane_row_sum<L>(L, [&](int p) { return w[p]; });
 }
 
 /* mask_row_label with the narrowest row width that covers the wave's
  * flagged rows (wave-uniform choice); all lanes of the wave call it */
 template <int L>
 __device__ __forceinline__ float mask_row_label_any(const int (&src)[L], const float (&w)[L],
-                                                    const float *Fo, int nullSlot, bool f,
-                                                    int len, float old) {
+                                                    float W, const float *Fo, int nullSlot,
+                                                    bool f, int len, float old) {
   static_assert(L == 8 || L == 16, "register rows of 8 or 16 records");
-  if (!__any(f && len > 4)) return mask_row_label<4, L>(src, w, Fo, nullSlot, f, old);
-  if (L == 8 || !__any(f && len > 8)) return mask_row_label<8, L>(src, w, Fo, nullSlot, f, old);
-  return mask_row_label<L, L>(src, w, Fo, nullSlot, f, old);
+  if (!__any(f && len > 4)) return mask_row_label<4, L>(src, w, W, Fo, nullSlot, f, old);
+  if (L == 8 || !__any(f && len > 8))
+    return mask_row_label<8, L>(src, w, W, Fo, nullSlot, f, old);
+  return mask_row_label<L, L>(src, w, W, Fo, nullSlot, f, old);
 }
 
 /* Single-wave owner mode: the work-group is one wave (64 threads) and the
@@ -936,6 +940,7 @@ __device__ __forceinline__ int wave_owner_iterations(
   float l0[V], l1[V];
   int len[V];
   unsigned outPos[V]; /* row positions naming eligible vertices outside the core */
+  float Wr[V];        /* weight sums of the rows */
   bool entered[V];
 #pragma unroll
   for (int r = 0; r < V; ++r) {
@@ -963,6 +968,7 @@ __device__ __forceinline__ int wave_owner_iterations(
         outPos[r] |= 1u << p;
       }
     }
+    Wr[r] = mask_row_weight<L>(w[r]);
   }
   __syncthreads();  // the flags are read
 
@@ -989,7 +995,7 @@ __device__ __forceinline__ int wave_owner_iterations(
     for (int r = 0; r < V; ++r) {
       const bool f = (fm[r] >> lane) & 1ull;
       const float old = kb ? l1[r] : l0[r];
-      const float nl = mask_row_label_any<L>(src[r], w[r], Fo, nullSlot, f, len[r], old);
+      const float nl = mask_row_label_any<L>(src[r], w[r], Wr[r], Fo, nullSlot, f, len[r], old);
       ch[r] = f && fabsf(nl - old) > a.delta;
       *(f ? &Fn[lane + 64 * r] : (float *)dumOut) = nl;
       l0[r] = (f && kb) ? nl : l0[r];
@@ -1132,6 +1138,7 @@ __device__ __forceinline__ int mask_owner_iterations(
       outPos |= 1u << p;
     }
   }
+  const float Wrow = mask_row_weight<L>(w);
   int n = nq;  // size of frontier k, known for the first iteration
   __syncthreads();  // the flags are read
 
@@ -1156,7 +1163,7 @@ __device__ __forceinline__ int mask_owner_iterations(
     const float old = kb ? l1 : l0;
     float nl = old;
     if (fm != 0ull)  // wave-uniform: a wave with no flagged vertex only syncs
-      nl = mask_row_label_any<L>(src, w, Fo, nullSlot, f, len, old);
+      nl = mask_row_label_any<L>(src, w, Wrow, Fo, nullSlot, f, len, old);
     const bool ch = f && fabsf(nl - old) > a.delta;
     *(f ? &Fn[tid] : (float *)dumOut) = nl;
     l0 = (f && kb) ? nl : l0;
