@@ -177,6 +177,8 @@ static inline int dlp_blocks(long long n, int per, int cap = 1 << 20) {
 /* ------------------------------------------------------------------------ */
 /* scratch sizes for n items */
 size_t dlp_scan_scratch_ints(long long n);
+/* scratch of the one-pass sparsification of n new edges */
+size_t dlp_sparsify_scratch_ints(long long n);
 /* exclusive scan of in[0..n) into out, total into *totalDev (device) */
 int dlp_scan_exclusive(const int *in, int *out, int n, int *totalDev, int *scratch,
                        hipStream_t s);

@@ -355,9 +355,12 @@ int dlp_export_marks(const DState &d, int tag, int *out, hipStream_t s);
 /* ------------------------------------------------------------------------ */
 int dlp_k_mark_deletions(int *rowPtr, int *colInd, int *status, int *isAffected,
                          const int *deleted, int numDeleted, int numNodes, hipStream_t s);
+/* scratch: dlp_sparsify_scratch_ints(numNewEdges) ints; errFlag gets bit 16
+ * if the one-pass compaction gave up waiting on a tile */
 int dlp_k_sparsify(const int *src, const int *dst, const float *w, int numNewEdges,
                    int numInserted, const int *isNew, float tau, int *isAffected, int *subSrc,
-                   int *subDst, float *subW, int *countDev, int *scratch, hipStream_t s);
+                   int *subDst, float *subW, int *countDev, int *scratch, int *errFlag,
+                   hipStream_t s);
 int dlp_k_fill_int(int *p, int value, long long n, hipStream_t s);
 int dlp_k_copy_float(float *dst, const float *src, long long n, hipStream_t s);
 /* ascending list of distinct inserted ids; its length goes to ctr->numNew */

@@ -334,7 +334,7 @@ int64_t dlp_ctx_scratch_layout(dlp_context *c, long long ins, long long e, char 
   const long long mi = std::max(ins, 1LL), me = std::max(e, 1LL);
   const long long scanInts = (long long)std::max(
       {dlp_scan_scratch_ints(d.numWords), dlp_scan_scratch_ints(me), dlp_scan_scratch_ints(mi),
-       dlp_scan_scratch_ints(N), dlp_radix_scratch_ints(mi)});
+       dlp_scan_scratch_ints(N), dlp_radix_scratch_ints(mi), dlp_sparsify_scratch_ints(me)});
   int64_t off = 0;
   auto carve = [&](auto **p, long long count) {
     using Ptr = std::remove_reference_t<decltype(*p)>;

@@ -165,12 +165,17 @@ extern "C" int dlp_build_sparsified_subgraph(
   if (rc != DLP_OK) return rc;
   hipStream_t s = (hipStream_t)stream;
   if ((rc = ws_reset_counters(*w, s)) != DLP_OK) return rc;
-  WS_BUF(int, scratch, 0, dlp_scan_scratch_ints(numNewEdges));
+  WS_BUF(int, scratch, 0, dlp_sparsify_scratch_ints(numNewEdges));
   rc = dlp_k_sparsify(newEdgeSrc, newEdgeDst, newEdgeWeight, numNewEdges, numInserted,
                       isNewVertex, tau, isAffected, subEdgeSrc, subColInd, subValues,
-                      &w->ctr->scanTotal, scratch, s);
+                      &w->ctr->scanTotal, scratch, &w->ctr->errorFlag, s);
   if (rc != DLP_OK) return rc;
   if ((rc = ws_read_counters(*w, s)) != DLP_OK) return rc;
+  if (w->pin->errorFlag) {
+    dlp_set_errorf_("dlp_build_sparsified_subgraph: device error flags 0x%x",
+                    w->pin->errorFlag);
+    return DLP_ERR_INTERNAL;
+  }
   *numSubEdges = 2 * w->pin->scanTotal;
   return DLP_OK;
 }

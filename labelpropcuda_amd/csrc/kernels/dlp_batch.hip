@@ -357,69 +357,170 @@ struct NewInts {
   __device__ bool operator()(int v) const { return flags[v] != 0; }
 };
 
-template <class IsNew>
-struct SpGet {
-  const int *src, *dst;
-  const float *w;
-  float tau;
-  IsNew isNew;
-  __device__ int operator()(int i) const {
-    return (w[i] > tau && isNew(src[i]) && isNew(dst[i])) ? 1 : 0;
-  }
-};
+/* One pass of LDS-staged stream compaction: tiles of kSpTile edges, one
+ * work-group each, taken in launch order from a device counter.  Row j of a
+ * tile is edges tileBase + 256 j + t (coalesced loads); a wave's kept edges
+ * of a row are packed by ballot and rank, the counts of every (row, wave)
+ * are exchanged and scanned in LDS, so each kept edge knows its place in
+ * the tile in input order, and the kept edges are staged there in that
+ * order.  The tile's offset among all tiles comes from a decoupled
+ * look-back over per-tile status words (aggregate, then inclusive prefix):
+ * a tile waits only on tiles that took their number before it and are
+ * running, and it gives up after kSpSpinMax polls (device error flag)
+ * rather than wait for ever.  The staged edges then leave as one contiguous
+ * run per tile, each as its two mirrored entries; every heavy edge's
+ * endpoints are marked; the last tile stores the number of kept edges. */
+constexpr int kSpRows = 8;
+constexpr int kSpTile = kSpRows * kT; /* 2048 edges */
+constexpr int kSpWaves = kT / 64;
+constexpr int kSpSpinMax = 1 << 22;
 
-/* pipeline emit: G' pair at 2*prefix, marks through the candidate list */
-struct SpEmitPipe {
-  SpGet<NewBits> g;
+__device__ __forceinline__ unsigned long long sp_status(unsigned flag, int value) {
+  return ((unsigned long long)flag << 62) | (unsigned)value;
+}
+
+/* pipeline output: mirrored pairs as 8-byte stores, candidate marks */
+struct SpOutPipe {
   int *subSrc, *subDst;
   float *subW;
   int numInserted;
   int *markStamp, *cand, *numCand;
   int markTag;
-  __device__ void operator()(int i, int prefix, bool valid) const {
-    int u = valid ? g.src[i] : -1, v = valid ? g.dst[i] : -1;
-    float w = valid ? g.w[i] : 0.0f;
-    const bool heavy = valid && !(w <= g.tau);
-    if (heavy && g.isNew(u) && g.isNew(v)) {
-      subSrc[2 * prefix] = u;
-      subDst[2 * prefix] = v;
-      subW[2 * prefix] = w;
-      subSrc[2 * prefix + 1] = v;
-      subDst[2 * prefix + 1] = u;
-      subW[2 * prefix + 1] = w;
-    }
-    dlp_mark_candidate(markStamp, markTag, cand, numCand, u, heavy && u >= 0 && u < numInserted);
-    dlp_mark_candidate(markStamp, markTag, cand, numCand, v, heavy && v >= 0 && v < numInserted);
+  __device__ void write(int q, int u, int v, float w) const {
+    reinterpret_cast<int2 *>(subSrc)[q] = make_int2(u, v);
+    reinterpret_cast<int2 *>(subDst)[q] = make_int2(v, u);
+    reinterpret_cast<float2 *>(subW)[q] = make_float2(w, w);
+  }
+  /* every lane of the wave calls it (wave-aggregated appends) */
+  __device__ void mark(int x, bool heavy) const {
+    dlp_mark_candidate(markStamp, markTag, cand, numCand, x,
+                       heavy && x >= 0 && x < numInserted);
   }
 };
 
-/* step-API emit: dst / weight arrays like the reference, optional sources,
- * isAffected marks */
-struct SpEmitStd {
-  SpGet<NewInts> g;
+/* step-API output (the reference's arrays: destinations and weights, sources
+ * optional; caller buffers, so 4-byte stores) and isAffected marks */
+struct SpOutStd {
   int *subSrc, *subDst;
   float *subW;
   int numInserted;
   int *isAffected;
-  __device__ void operator()(int i, int prefix, bool valid) const {
-    if (!valid) return;
-    int u = g.src[i], v = g.dst[i];
-    float w = g.w[i];
-    if (w <= g.tau) return;
-    if (g.isNew(u) && g.isNew(v)) {
-      if (subSrc) {
-        subSrc[2 * prefix] = u;
-        subSrc[2 * prefix + 1] = v;
-      }
-      subDst[2 * prefix] = v;
-      subW[2 * prefix] = w;
-      subDst[2 * prefix + 1] = u;
-      subW[2 * prefix + 1] = w;
+  __device__ void write(int q, int u, int v, float w) const {
+    if (subSrc) {
+      subSrc[2 * q] = u;
+      subSrc[2 * q + 1] = v;
     }
-    if (u >= 0 && u < numInserted) atomicMax(&isAffected[u], 1);
-    if (v >= 0 && v < numInserted) atomicMax(&isAffected[v], 1);
+    subDst[2 * q] = v;
+    subDst[2 * q + 1] = u;
+    subW[2 * q] = w;
+    subW[2 * q + 1] = w;
+  }
+  __device__ void mark(int x, bool heavy) const {
+    if (heavy && x >= 0 && x < numInserted) atomicMax(&isAffected[x], 1);
   }
 };
+
+template <class IsNew, class Out>
+__global__ __launch_bounds__(kT) void k_sparsify_pass(const int *src, const int *dst,
+                                                      const float *wt, int n, float tau,
+                                                      IsNew isNew, Out out,
+                                                      unsigned long long *tileState,
+                                                      int *tileCounter, int *total, int *err) {
+  __shared__ int cnt[kSpRows * kSpWaves]; /* kept edges per (row, wave), then offsets */
+  __shared__ int tileInfo[3];             /* tile number, tile offset, kept edges */
+  __shared__ int stU[kSpTile], stV[kSpTile];
+  __shared__ float stW[kSpTile];          /* the tile's kept edges, in input order */
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  if (t == 0) tileInfo[0] = atomicAdd(tileCounter, 1);
+  __syncthreads();
+  const int tile = tileInfo[0];
+  const int tiles = (int)(((long long)n + kSpTile - 1) / kSpTile);
+  const long long base = (long long)tile * kSpTile;
+  int u[kSpRows], v[kSpRows];
+  float w[kSpRows];
+  unsigned long long keep[kSpRows];
+  bool heavy[kSpRows];
+#pragma unroll
+  for (int j = 0; j < kSpRows; ++j) {
+    const long long i = base + (long long)j * kT + t;
+    const bool valid = i < n;
+    u[j] = valid ? src[i] : -1;
+    v[j] = valid ? dst[i] : -1;
+    w[j] = valid ? wt[i] : 0.0f;
+    /* the reference's test, !(w <= tau) (sparsifySubgraph.cu:163), keeps a
+     * NaN weight too */
+    heavy[j] = valid && !(w[j] <= tau);
+    keep[j] = dlp_ballot(heavy[j] && isNew(u[j]) && isNew(v[j]));
+    if (lane == 0) cnt[j * kSpWaves + wave] = __popcll(keep[j]);
+  }
+  __syncthreads();
+  if (wave == 0) {
+    /* exclusive offsets of the (row, wave) groups in input order */
+    constexpr int G = kSpRows * kSpWaves; /* 32 */
+    const int x = lane < G ? cnt[lane] : 0;
+    int incl = x;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int y = __shfl_up(incl, o, 64);
+      if (lane >= o) incl += y;
+    }
+    if (lane < G) cnt[lane] = incl - x;
+    const int agg = __shfl(incl, G - 1, 64);
+    /* publish the aggregate, look back for the prefix, publish the prefix */
+    if (lane == 0)
+      __hip_atomic_store(&tileState[tile], sp_status(tile == 0 ? 2u : 1u, agg), __ATOMIC_RELEASE,
+                         __HIP_MEMORY_SCOPE_AGENT);
+    int excl = 0;
+    int p = tile - 1; /* newest predecessor not yet summed */
+    int spins = 0;
+    while (p >= 0) {  // wave-uniform
+      const int q = p - lane;
+      const unsigned long long st =
+          q >= 0 ? __hip_atomic_load(&tileState[q], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT)
+                 : sp_status(2u, 0);
+      const unsigned flag = (unsigned)(st >> 62);
+      const unsigned long long incM = dlp_ballot(flag == 2u);
+      const int first = incM ? __ffsll((long long)incM) - 1 : 64; /* nearest prefix */
+      const unsigned long long upTo = first >= 63 ? ~0ull : ((2ull << first) - 1ull);
+      if (dlp_ballot(flag == 0u) & upTo) {  // a predecessor has published nothing yet
+        if (++spins > kSpSpinMax) {
+          if (lane == 0) atomicOr(err, 16);
+          break;
+        }
+        continue;
+      }
+      int val = lane <= first ? (int)(unsigned)st : 0;
+#pragma unroll
+      for (int o = 32; o >= 1; o >>= 1) val += __shfl_xor(val, o, 64);
+      excl += val;
+      if (first < 64) break;
+      p -= 64;
+    }
+    if (lane == 0) {
+      if (tile > 0)
+        __hip_atomic_store(&tileState[tile], sp_status(2u, excl + agg), __ATOMIC_RELEASE,
+                           __HIP_MEMORY_SCOPE_AGENT);
+      if (tile == tiles - 1) *total = excl + agg;
+      tileInfo[1] = excl;
+      tileInfo[2] = agg;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < kSpRows; ++j) {
+    if ((keep[j] >> lane) & 1ull) {
+      const int pos = cnt[j * kSpWaves + wave] + dlp_rank_below(keep[j]);
+      stU[pos] = u[j];
+      stV[pos] = v[j];
+      stW[pos] = w[j];
+    }
+    out.mark(u[j], heavy[j]);
+    out.mark(v[j], heavy[j]);
+  }
+  __syncthreads();
+  const int off = tileInfo[1], kept = tileInfo[2];
+  for (int p = t; p < kept; p += kT) out.write(off + p, stU[p], stV[p], stW[p]);
+}
 
 /* ------------------------------------------------------------------------ */
 /* connected components (both paths)                                        */
@@ -914,11 +1015,28 @@ int dlp_sorted_inserted(const DState &d, hipStream_t s) {
   return dlp_bitmap_to_list(d.newBits, d.numWords, d.S, &d.ctr->numNew, d.scratch, s);
 }
 
+/* the one-pass compaction over n edges; scratch holds
+ * dlp_sparsify_scratch_ints(n) ints (tile status words, tile counter) */
+template <class IsNew, class Out>
+int sparsify_launch(const int *src, const int *dst, const float *w, int n, float tau,
+                    IsNew isNew, Out out, int *total, int *scratch, int *err, hipStream_t s) {
+  const int tiles = (int)std::max<long long>(1, ((long long)n + kSpTile - 1) / kSpTile);
+  unsigned long long *state = reinterpret_cast<unsigned long long *>(scratch);
+  int *counter = scratch + 2 * tiles;
+  hipLaunchKernelGGL(k_fill_int, dim3(dlp_blocks(2LL * tiles + 1, kT)), dim3(kT), 0, s, scratch,
+                     0, 2LL * tiles + 1);
+  DLP_LAUNCHED("k_fill_int");
+  hipLaunchKernelGGL((k_sparsify_pass<IsNew, Out>), dim3(tiles), dim3(kT), 0, s, src, dst, w, n,
+                     tau, isNew, out, state, counter, total, err);
+  DLP_LAUNCHED("k_sparsify_pass");
+  return DLP_OK;
+}
+
 int dlp_sparsify(const DState &d, const DBatch &bt, float tau, int markTag, hipStream_t s) {
-  SpGet<NewBits> g{bt.eSrc, bt.eDst, bt.eW, tau, NewBits{d.newBits}};
-  SpEmitPipe e{g, d.subSrc, d.subDst, d.subW, bt.nIns, d.markStamp, d.cand, &d.ctr->numCand,
-               markTag};
-  return dlp::scan_launch(g, e, bt.nE, &d.ctr->subPairs, d.scratch, s);
+  SpOutPipe out{d.subSrc, d.subDst, d.subW, bt.nIns, d.markStamp, d.cand, &d.ctr->numCand,
+                markTag};
+  return sparsify_launch(bt.eSrc, bt.eDst, bt.eW, bt.nE, tau, NewBits{d.newBits}, out,
+                         &d.ctr->subPairs, d.scratch, &d.ctr->errorFlag, s);
 }
 
 int dlp_components(const DState &d, const DBatch &bt, hipStream_t s) {
@@ -1029,6 +1147,11 @@ int dlp_export_marks(const DState &d, int tag, int *out, hipStream_t s) {
 /* ========================================================================== */
 /* step API launchers                                                         */
 /* ========================================================================== */
+size_t dlp_sparsify_scratch_ints(long long n) {
+  const long long tiles = std::max<long long>(1, (n + kSpTile - 1) / kSpTile);
+  return (size_t)(2 * tiles + 64);
+}
+
 int dlp_k_fill_int(int *p, int value, long long n, hipStream_t s) {
   if (n <= 0) return DLP_OK;
   hipLaunchKernelGGL(k_fill_int, dim3(grid_for(n)), dim3(kT), 0, s, p, value, n);
@@ -1057,10 +1180,11 @@ int dlp_k_mark_deletions(int *rowPtr, int *colInd, int *status, int *isAffected,
 
 int dlp_k_sparsify(const int *src, const int *dst, const float *w, int numNewEdges,
                    int numInserted, const int *isNew, float tau, int *isAffected, int *subSrc,
-                   int *subDst, float *subW, int *countDev, int *scratch, hipStream_t s) {
-  SpGet<NewInts> g{src, dst, w, tau, NewInts{isNew}};
-  SpEmitStd e{g, subSrc, subDst, subW, numInserted, isAffected};
-  return dlp::scan_launch(g, e, numNewEdges, countDev, scratch, s);
+                   int *subDst, float *subW, int *countDev, int *scratch, int *errFlag,
+                   hipStream_t s) {
+  SpOutStd out{subSrc, subDst, subW, numInserted, isAffected};
+  return sparsify_launch(src, dst, w, numNewEdges, tau, NewInts{isNew}, out, countDev, scratch,
+                         errFlag, s);
 }
 
 int dlp_k_sorted_list(const int *inserted, int numInserted, int numNodes, uint32_t *bits,
