@@ -382,6 +382,10 @@ int dlp_context_set_counter_bases(dlp_context *ctx, int stampBase, int localGen)
  * receives 1 if the full graph names v in u's row exactly as often as u in
  * v's row.  The CSR, and everything after it, is the same in every mode. */
 int dlp_context_row_update(dlp_context *ctx, int mode, int *symmetric);
+/* The same symmetry test on a host CSR, without a GPU: *symmetric = 1 if
+ * the entries with column >= 0, as (row, column) pairs, form the same
+ * multiset as their reverses (no counterpart in the reference). */
+int dlp_graph_symmetric(const dlp_graph_view *graph, int *symmetric);
 /* Current device values of both counters (synchronises). */
 int dlp_context_counter_bases(dlp_context *ctx, int *stampBase, int *localGen);
 

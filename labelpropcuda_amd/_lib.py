@@ -265,6 +265,7 @@ SIGNATURES = {
     "dlp_multi_run_next": (C.c_int, [_VP, C.POINTER(C.c_int), C.c_int]),
     "dlp_multi_set_scratch_pool": (C.c_int, [_VP, C.c_int64]),
     "dlp_context_row_update": (C.c_int, [_VP, C.c_int, C.POINTER(C.c_int)]),
+    "dlp_graph_symmetric": (C.c_int, [C.POINTER(GraphView), C.POINTER(C.c_int)]),
     "dlp_multi_pool_stats": (C.c_int, [_VP, C.POINTER(PoolStats)]),
     "dlp_multi_batch_scratch_bytes": (C.c_int, [_VP, C.c_int, C.c_int, C.POINTER(C.c_int64)]),
     "dlp_multi_serve": (C.c_int, [_VP, C.POINTER(ServeRequest), C.c_int, C.POINTER(ServeConfig),

@@ -820,6 +820,21 @@ int dlp_ctx_counter_bases(dlp_context *c, int *stampBase, int *localGen) {
   return DLP_OK;
 }
 
+extern "C" int dlp_graph_symmetric(const dlp_graph_view *g, int *symmetric) {
+  if (!g || !symmetric || g->numNodes < 0 || g->numEdges < 0 ||
+      (g->numNodes > 0 && !g->rowPtr) || (g->numEdges > 0 && !g->colInd)) {
+    dlp_set_error_("dlp_graph_symmetric: bad argument");
+    return DLP_ERR_INVALID;
+  }
+  for (int u = 0; u < g->numNodes; ++u)
+    if (g->rowPtr[u] > g->rowPtr[u + 1] || g->rowPtr[u + 1] > g->numEdges) {
+      dlp_set_errorf_("dlp_graph_symmetric: rowPtr not monotone within numEdges at row %d", u);
+      return DLP_ERR_INVALID;
+    }
+  *symmetric = csr_symmetric(g) ? 1 : 0;
+  return DLP_OK;
+}
+
 extern "C" int dlp_context_row_update(dlp_context *c, int mode, int *symmetric) {
   if (!c || mode > 2) {
     dlp_set_error_("dlp_context_row_update: NULL context or mode not in {0, 1, 2}");

@@ -609,6 +609,20 @@ class Multi:
         return done[:n], steps.value
 
 
+def graph_symmetric(rowPtr, colInd) -> bool:
+    """Whether a host CSR names v in u's row exactly as often as u in v's row
+    (entries < 0 left out); no GPU needed."""
+    rp = np.ascontiguousarray(rowPtr, np.int32)
+    ci = np.ascontiguousarray(colInd, np.int32)
+    vals = np.zeros(max(len(ci), 1), np.float32)
+    g = _lib.GraphView()
+    g.numNodes, g.numEdges = len(rp) - 1, len(ci)
+    g.rowPtr, g.colInd, g.values = _ptr(rp, C.c_int), _ptr(ci, C.c_int), _ptr(vals, C.c_float)
+    out = C.c_int()
+    check(load().dlp_graph_symmetric(C.byref(g), C.byref(out)))
+    return bool(out.value)
+
+
 def autotune_heuristic(inputs: Inputs):
     """The degree-histogram heuristic for a graph: (shape key, tuning)."""
     k, t = _lib.ShapeKey(), _lib.Tuning()
