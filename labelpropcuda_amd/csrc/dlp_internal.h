@@ -71,7 +71,7 @@ struct DevCounters {
   unsigned long long profLocalWaveUpdates;  /* of profLocalOwnerUpdates: in the
                                                single-wave owner kernel      */
   int rowCount;        /* rows listed for the batch's CSR rebuild           */
-  int pad2;
+  int accFull;         /* the batch's accuracy came from a whole-graph pass */
   /* accuracy numerator / denominator over all vertices, kept up to date
    * from the vertices each batch changes (zeroed by a reset only) */
   unsigned long long accCorrect;

@@ -795,36 +795,59 @@ __global__ void k_enforce_gt(DState d, DBatch bt) {
 }
 
 /* computeAccuracy (dynLP.cu:65-85): over the vertices whose host status is
- * ACTIVE, the share whose label (>= 0.5) gives the true class.  Kept as
- * running totals over all vertices: a vertex's contribution (accState)
- * changes only when its host status or its label does, i.e. for the
- * vertices of the batch's lists and those propagation wrote (the touched
- * list), so only they are visited.  atomicExch makes a vertex listed twice
- * count its change once. */
+ * ACTIVE, the share whose label (>= 0.5) gives the true class.  Each
+ * vertex's contribution is kept (accState) with running totals over all
+ * vertices.  A contribution changes only when the vertex's host status or
+ * its label does, i.e. for the batch's listed vertices and those
+ * propagation wrote (the touched list); when those are few (at most N/16)
+ * only they are visited and their changes added to the totals, atomicExch
+ * making a vertex listed twice count once.  Otherwise a coalesced pass over
+ * all vertices is cheaper than that many scattered visits: it rewrites
+ * every state and counts the batch's totals afresh (k_batch_record then
+ * takes them as the running totals). */
+__device__ __forceinline__ int acc_state(const DState &d, int v) {
+  if (!(d.vflags[v] & VF_HSACT)) return 0;
+  const int pred = d.lab0[v] >= 0.5f ? 1 : 0;
+  const int truth = d.trueLab[v] >= 0.5f ? 1 : 0;
+  return 1 | (pred == truth ? 2 : 0);
+}
+
 __global__ __launch_bounds__(kT) void k_accuracy_delta(DState d, DBatch bt) {
   __shared__ int lds[4];
   DevCounters *c = d.ctr;
   const long long n1 = bt.nIns, n2 = n1 + bt.nDel, n3 = n2 + bt.nGt, n4 = n3 + c->touched;
-  int dCorr = 0, dTot = 0;
-  for (long long i = (long long)blockIdx.x * kT + threadIdx.x; i < n4;
-       i += (long long)gridDim.x * kT) {
-    const int v = i < n1 ? bt.ins[i]
-                         : (i < n2 ? bt.del[i - n1] : (i < n3 ? bt.gtIds[i - n2] : d.touched[i - n3]));
-    int st = 0;
-    if (d.vflags[v] & VF_HSACT) {
-      const int pred = d.lab0[v] >= 0.5f ? 1 : 0;
-      const int truth = d.trueLab[v] >= 0.5f ? 1 : 0;
-      st = 1 | (pred == truth ? 2 : 0);
+  const bool full = 16 * n4 > (long long)d.N;  // uniform
+  int corr = 0, tot = 0;
+  if (full) {
+    for (long long v = (long long)blockIdx.x * kT + threadIdx.x; v < d.N;
+         v += (long long)gridDim.x * kT) {
+      const int st = acc_state(d, (int)v);
+      d.accState[v] = st;
+      corr += st == 3;
+      tot += st & 1;
     }
-    const int old = atomicExch(&d.accState[v], st);
-    dCorr += (st == 3) - (old == 3);
-    dTot += (st & 1) - (old & 1);
+  } else {
+    for (long long i = (long long)blockIdx.x * kT + threadIdx.x; i < n4;
+         i += (long long)gridDim.x * kT) {
+      const int v = i < n1 ? bt.ins[i]
+                           : (i < n2 ? bt.del[i - n1]
+                                     : (i < n3 ? bt.gtIds[i - n2] : d.touched[i - n3]));
+      const int st = acc_state(d, v);
+      const int old = atomicExch(&d.accState[v], st);
+      corr += (st == 3) - (old == 3);
+      tot += (st & 1) - (old & 1);
+    }
   }
-  dCorr = dlp::block_sum(dCorr, lds);
-  dTot = dlp::block_sum(dTot, lds);
-  if (threadIdx.x == 0) {  // two's complement: a negative change adds as its wrap-around
-    if (dCorr) atomicAdd(&c->accCorrect, (unsigned long long)(long long)dCorr);
-    if (dTot) atomicAdd(&c->accTotal, (unsigned long long)(long long)dTot);
+  corr = dlp::block_sum(corr, lds);
+  tot = dlp::block_sum(tot, lds);
+  if (threadIdx.x == 0) {
+    if (blockIdx.x == 0) c->accFull = full ? 1 : 0;
+    /* full: this batch's counts (zeroed per batch); else changes of the
+     * running totals (a negative change adds as its two's complement) */
+    unsigned long long *cc = full ? &c->correct : &c->accCorrect;
+    unsigned long long *tt = full ? &c->total : &c->accTotal;
+    if (corr) atomicAdd(cc, (unsigned long long)(long long)corr);
+    if (tot) atomicAdd(tt, (unsigned long long)(long long)tot);
   }
 }
 
@@ -838,8 +861,13 @@ __global__ void k_batch_record(DState d, int b) {
   r.iterations = c->iterUsed;
   r.touched = c->touched;
   r.updates = c->updates;
-  c->correct = c->accCorrect;
-  c->total = c->accTotal;
+  if (c->accFull) {  // the whole-graph pass counted afresh
+    c->accCorrect = c->correct;
+    c->accTotal = c->total;
+  } else {
+    c->correct = c->accCorrect;
+    c->total = c->accTotal;
+  }
   r.correct = c->accCorrect;
   r.total = c->accTotal;
   d.records[b] = r;

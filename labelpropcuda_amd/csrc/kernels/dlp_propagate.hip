@@ -106,15 +106,18 @@ constexpr int kClaimFilterMin = 16384;
  * than this many in one work-group go to the wave path */
 constexpr int kHubCap = 64;
 
-template <int G>
+/* HUB: the work-group tier for rows longer than blockRowMin is compiled in
+ * (a separate instantiation, launched only when the tier is on, so the
+ * common kernel carries none of its LDS or code) */
+template <int G, bool HUB>
 __global__ __launch_bounds__(kPT) void k_propagate(PropArgs a, int k) {
   constexpr int GPW = 64 / G; /* frontier vertices per wave */
   __shared__ int stageQ[kStageQ];
   __shared__ int stageT[kStageT];
   __shared__ int stageCount[2], stageBase[2];
-  __shared__ int hubList[kHubCap];
+  __shared__ int hubList[HUB ? kHubCap : 1];
   __shared__ int hubCount, hubChanged;
-  __shared__ float hubW[256], hubS[256];
+  __shared__ float hubW[HUB ? 256 : 1], hubS[HUB ? 256 : 1];
   DevCounters *c = a.ctr;
   const int *in = (k & 1) ? a.q1 : a.q0;
   const int lane = threadIdx.x & 63;
@@ -168,19 +171,22 @@ __global__ __launch_bounds__(kPT) void k_propagate(PropArgs a, int k) {
       }
     }
     /* hub rows (longer than blockRowMin) wait for the whole work-group
-     * after this loop; a full hub list leaves them to the wave path */
-    bool deferred = false;
-    if (act && gl == 0 && a.blockRowMin > 0 && len > a.blockRowMin) {
-      const int slot = atomicAdd(&hubCount, 1);
-      if (slot < kHubCap) {
-        hubList[slot] = u;
-        deferred = true;
+     * after this loop; a full hub list leaves them to the wave path.  Off
+     * (blockRowMin 0, uniform) it costs nothing: the group learns its
+     * leader's decision by a ballot, not a shuffle in the row's chain. */
+    if (HUB) {
+      bool deferred = false;
+      if (act && gl == 0 && len > a.blockRowMin) {
+        const int slot = atomicAdd(&hubCount, 1);
+        if (slot < kHubCap) {
+          hubList[slot] = u;
+          deferred = true;
+        }
       }
-    }
-    deferred = __shfl(deferred ? 1 : 0, grp * G, 64) != 0;
-    if (deferred) {
-      act = false;
-      len = 0;
+      if ((dlp_ballot(deferred) >> (grp * G)) & 1ull) {
+        act = false;
+        len = 0;
+      }
     }
     /* rows of up to waveRowMin (<= 64) entries by the lane group; longer
      * ones (rare at the degrees of these graphs) by the whole wave, one
@@ -243,7 +249,7 @@ __global__ __launch_bounds__(kPT) void k_propagate(PropArgs a, int k) {
   __syncthreads();
   /* hub rows: the whole work-group sums each (the reference's 256-slot
    * order) and claims its neighbours */
-  const int nHub = min(hubCount, kHubCap);
+  const int nHub = HUB ? min(hubCount, kHubCap) : 0;
   for (int h = 0; h < nHub; ++h) {  // block-uniform
     const int u = hubList[h];
     int beg, len;
@@ -255,9 +261,10 @@ __global__ __launch_bounds__(kPT) void k_propagate(PropArgs a, int k) {
       beg = a.rowBeg[u];
       len = a.rowLen ? a.rowLen[u] : a.rowEnd[u] - beg;
     }
-    float W, S;
-    dlp::block_row_sums(len, GlobalPos{a.colInd, a.values, Fold, a.status, beg}, hubW, hubS, W,
-                        S);
+    float W = 0.0f, S = 0.0f;
+    if constexpr (HUB)
+      dlp::block_row_sums(len, GlobalPos{a.colInd, a.values, Fold, a.status, beg}, hubW, hubS, W,
+                          S);
     if (threadIdx.x == 0) {
       const float oldLabel = Fold[u];
       const float nl = dlp::new_label(W, S, oldLabel);
@@ -2004,12 +2011,21 @@ __global__ __launch_bounds__(LT) void k_propagate_multi(const GraphProp *graphs)
 
 int dlp_launch_propagate(const PropArgs &a, int k, int groupLanes, int blocks, hipStream_t s) {
   dim3 grid(blocks > 0 ? blocks : 1), block(kPT);
+  const bool hub = a.blockRowMin > 0;
+#define DLP_LAUNCH_PROP(Gl)                                                           \
+  case Gl:                                                                            \
+    if (hub)                                                                          \
+      hipLaunchKernelGGL((k_propagate<Gl, true>), grid, block, 0, s, a, k);           \
+    else                                                                              \
+      hipLaunchKernelGGL((k_propagate<Gl, false>), grid, block, 0, s, a, k);          \
+    break;
   switch (groupLanes) {
-    case 4: hipLaunchKernelGGL(k_propagate<4>, grid, block, 0, s, a, k); break;
-    case 8: hipLaunchKernelGGL(k_propagate<8>, grid, block, 0, s, a, k); break;
-    case 16: hipLaunchKernelGGL(k_propagate<16>, grid, block, 0, s, a, k); break;
-    case 32: hipLaunchKernelGGL(k_propagate<32>, grid, block, 0, s, a, k); break;
-    case 64: hipLaunchKernelGGL(k_propagate<64>, grid, block, 0, s, a, k); break;
+    DLP_LAUNCH_PROP(4)
+    DLP_LAUNCH_PROP(8)
+    DLP_LAUNCH_PROP(16)
+    DLP_LAUNCH_PROP(32)
+    DLP_LAUNCH_PROP(64)
+#undef DLP_LAUNCH_PROP
     default:
       dlp_set_errorf_("propagation group size %d not in {4,8,16,32,64}", groupLanes);
       return DLP_ERR_INVALID;
