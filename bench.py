@@ -86,6 +86,23 @@ def random_access_ceilings(array_ints: int, accesses: int = 1 << 24) -> dict:
             "store_per_s": rates[2]}
 
 
+def grid_sync_costs(blocks=(64, 512), rounds: int = 2000) -> list:
+    """Per-round cost of a device-wide sync between dependent rounds of work
+    (dlp_grid_sync_cost): a launch boundary against an in-kernel barrier of
+    one counter or of per-work-group flags, at the multi-block kernel's
+    smallest grid (512 work-groups) and a small one."""
+    import ctypes as C
+    from labelpropcuda_amd import _lib
+    out = []
+    for nb in blocks:
+        us, ok = (C.c_double * 3)(), (C.c_int * 3)()
+        _lib.check(_lib.load().dlp_grid_sync_cost(nb, rounds, us, ok))
+        out.append({"blocks": nb, "rounds": rounds, "launch_us": us[0],
+                    "counter_barrier_us": us[1], "flag_barrier_us": us[2],
+                    "exchange_ok": bool(ok[0] and ok[1] and ok[2])})
+    return out
+
+
 def random_access_floor_us(prof: dict, updates: int, launches: float, ra: dict) -> float:
     """Time per multi-block launch if its random accesses ran at the measured
     rates back to back: per vertex update the row start, row length and old
@@ -841,6 +858,7 @@ def main() -> int:
                      "algorithmic_bytes_per_launch": dom["algorithmic_bytes_per_launch"]},
         "propagation_kernels": engines,
         "random_access": ra_out.get("random_access"),
+        "grid_sync": grid_sync_costs(),
         "tuning": tuning,
     }
     print(f"[bench] C2 done in {time.perf_counter() - t_start:.1f} s: {value / 1e6:.1f} M "

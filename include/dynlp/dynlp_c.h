@@ -559,6 +559,14 @@ int dlp_copy_device(void *dst, const void *src, size_t bytes);
  * loads, rates[2] stores -- the ceilings of the gather- and atomic-bound
  * propagation kernels (no counterpart in the reference). */
 int dlp_random_access_rates(int64_t arrayInts, int64_t accesses, double *rates);
+/* Cost of a device-wide sync between two dependent rounds of work on
+ * `blocks` work-groups of 512 threads (each round: read the word another
+ * work-group wrote in the round before, write one's own): us[0] per round
+ * with one launch per round, us[1] per round inside one cooperative launch
+ * with a bounded counter barrier between rounds, us[2] the same with a
+ * barrier of per-work-group flags; ok[i] = 1 when every round saw all of
+ * the previous round's writes (no counterpart in the reference). */
+int dlp_grid_sync_cost(int blocks, int rounds, double *us, int *ok);
 
 /* ====================================================================== */
 /* 7. IrLP: the full-recompute baseline (validation.h, main.cu:120-217)  */

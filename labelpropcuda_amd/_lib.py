@@ -275,6 +275,8 @@ SIGNATURES = {
                                              C.c_float]),
     "dlp_copy_device": (C.c_int, [_VP, _VP, C.c_size_t]),
     "dlp_random_access_rates": (C.c_int, [C.c_int64, C.c_int64, C.POINTER(C.c_double)]),
+    "dlp_grid_sync_cost": (C.c_int, [C.c_int, C.c_int, C.POINTER(C.c_double),
+                                     C.POINTER(C.c_int)]),
     "dlp_write_graph_files": (C.c_int, [C.POINTER(GraphView), f32p, C.c_char_p]),
     "dlp_inputs_from_graph_files": (C.c_int, [C.POINTER(_VP), C.c_char_p, C.c_int, C.c_uint,
                                               C.c_float, C.c_float]),
