@@ -710,6 +710,9 @@ def main() -> int:
                     help="owner mode runs in the register-mask LDS kernels on frontiers of at "
                          "most this many vertices (0 = default, -1 = never)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--grid-sync", action="store_true",
+                    help="c2: add the device-wide sync costs (launch boundary against "
+                         "in-kernel barriers; cooperative launches, so not in the default line)")
     ap.add_argument("--workload", choices=("c2", "batched", "serving", "irlp", "stream"),
                     default="c2",
                     help="c2: one 500,000-vertex graph (default); batched: many graphs, "
@@ -858,9 +861,10 @@ def main() -> int:
                      "algorithmic_bytes_per_launch": dom["algorithmic_bytes_per_launch"]},
         "propagation_kernels": engines,
         "random_access": ra_out.get("random_access"),
-        "grid_sync": grid_sync_costs(),
         "tuning": tuning,
     }
+    if args.grid_sync:
+        out["grid_sync"] = grid_sync_costs()
     print(f"[bench] C2 done in {time.perf_counter() - t_start:.1f} s: {value / 1e6:.1f} M "
           "updates/s", file=sys.stderr, flush=True)
     if not args.no_cpu:
