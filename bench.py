@@ -35,6 +35,15 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+# Serving finishes handed-back graphs on host threads with a stream each (8
+# by default) beside the serving loop's stream.  HIP maps a process's
+# streams onto GPU_MAX_HW_QUEUES hardware queues (4 unless set), and kernels
+# of streams that share a queue run one after another, so a lane's long
+# LDS-resident launch would hold up the shared serving launches queued
+# behind it.  16 queues give each of these streams its own; the variable is
+# read when the HIP runtime starts, so it is set before anything loads it.
+os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
+
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 
 
@@ -390,7 +399,8 @@ def serve_run(m, tenants, reqTenant, arrivals, plain, args, keep_labels=False):
     reqs = list(zip(reqTenant.tolist(), arrivals.tolist()))
     done, steps = m.serve(reqs, maxSlots=args.slots, hbmBudgetBytes=args.hbm_budget,
                           plain=plain, handBackLanes=args.lanes_back,
-                          sliceIterations=args.slice_iterations, sliceMs=args.slice_ms)
+                          sliceIterations=args.slice_iterations, sliceMs=args.slice_ms,
+                          handBackAfter=args.hand_back_after)
     m.synchronize()
     lat = 1000.0 * (done - arrivals)
     assert np.all(lat >= 0.0), "a request completed before it arrived"
@@ -464,6 +474,7 @@ def serving_leg(args, progress, sweep_rates) -> dict:
                     "batches of log-uniform 100-100000 vertices (90/1/9)",
         "slots": args.slots, "hbm_budget_bytes": args.hbm_budget,
         "slice_iterations": args.slice_iterations, "slice_ms": args.slice_ms,
+        "hand_back_after": args.hand_back_after,
         "updates": sat_served["updates"],
         "saturated": {"served": sat_served, "plain": sat_plain,
                       "speedup_vs_plain": sat_served["updates_per_s"] /
@@ -745,6 +756,9 @@ def main() -> int:
     ap.add_argument("--graphs", type=int, default=256, help="graphs of the batched workload")
     ap.add_argument("--heavy-sweep", type=int, nargs="*", default=[-1, 65536, 16384, 4096],
                     help="batched: hand-back frontier sizes the autotuner tries")
+    ap.add_argument("--hand-back-after", type=int, default=0,
+                    help="serving: a batch still running at this iteration goes to a "
+                         "hand-back lane (0 = default 1024, -1 = never)")
     ap.add_argument("--lanes-back", type=int, default=0,
                     help="serving: hand-back lanes (0 = library default)")
     ap.add_argument("--repeats", type=int, default=3,

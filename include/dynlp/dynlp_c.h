@@ -524,6 +524,12 @@ typedef struct dlp_serve_config {
   double sliceMs;         /* ... or after this many milliseconds of its
                              work-group HBM iterations (0 = 2.0, < 0 =
                              no limit); the next step resumes it       */
+  int handBackAfter;      /* with hand-back on, a graph stopped by its
+                             slice at or past this iteration of its
+                             batch goes to a hand-back lane, whose
+                             owner-mode LDS kernels run a long-lived
+                             frontier (a limit cycle) faster than the
+                             shared launch (0 = 1024, -1 = never)      */
 } dlp_serve_config;
 
 /* Arrival-driven continuous batching with iteration-level admission:
@@ -536,7 +542,8 @@ typedef struct dlp_serve_config {
  * deferred; a batch larger than the whole pool is an error, DLP_ERR_NOMEM)
  * -- and join the running ones.  completion[i] receives the time (seconds
  * from the start) at which request i's labels were ready.  With hand-back
- * on, a graph whose frontier outgrows heavyFrontier leaves the shared
+ * on, a graph whose frontier outgrows heavyFrontier, or that is still
+ * running at iteration handBackAfter of its batch, leaves the shared
  * launches and is finished by a hand-back lane while the steps go on; its
  * tenant's next request waits for it.  Stopping and resuming a graph does
  * not change its iterations or labels. */

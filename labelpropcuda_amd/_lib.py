@@ -143,7 +143,7 @@ class ServeRequest(C.Structure):
 class ServeConfig(C.Structure):
     _fields_ = [("maxSlots", C.c_int), ("hbmBudgetBytes", C.c_int64), ("plain", C.c_int),
                 ("handBackLanes", C.c_int), ("sliceIterations", C.c_int),
-                ("sliceMs", C.c_double)]
+                ("sliceMs", C.c_double), ("handBackAfter", C.c_int)]
 
 
 class PoolStats(C.Structure):

@@ -1097,6 +1097,12 @@ namespace {
  * resumes it beside the batches admitted in between */
 constexpr int kServeSliceIterations = 2048;
 constexpr double kServeSliceMs = 2.0;
+/* default dlp_serve_config::handBackAfter: a batch still running after this
+ * many iterations is in a long-lived frontier (a limit cycle runs to the
+ * 100,000-iteration cap), which the single-graph loop's owner-mode LDS
+ * kernels iterate several times faster than a work-group of the shared
+ * launch in queue mode */
+constexpr int kServeHandBackAfter = 1024;
 
 /* a batch in the shared propagation launches of dlp_multi_serve */
 struct Flight {
@@ -1109,7 +1115,8 @@ struct Flight {
 extern "C" int dlp_multi_serve(dlp_multi *m, const dlp_serve_request *reqs, int numReqs,
                                const dlp_serve_config *cfg, double *completion, int *steps) {
   if (!m || numReqs < 0 || (numReqs > 0 && (!reqs || !completion)) || !cfg ||
-      cfg->maxSlots < 1 || cfg->hbmBudgetBytes < 0 || cfg->sliceIterations < -1) {
+      cfg->maxSlots < 1 || cfg->hbmBudgetBytes < 0 || cfg->sliceIterations < -1 ||
+      cfg->handBackAfter < -1) {
     dlp_set_error_("dlp_multi_serve: bad argument");
     return DLP_ERR_INVALID;
   }
@@ -1141,6 +1148,7 @@ extern "C" int dlp_multi_serve(dlp_multi *m, const dlp_serve_request *reqs, int 
   const double sliceMs = cfg->sliceMs == 0.0 ? kServeSliceMs : cfg->sliceMs;
   const long long sliceTicks =
       sliceMs > 0.0 && m->wallRateKHz > 0 ? (long long)(sliceMs * m->wallRateKHz) : 0;
+  const int longAfter = cfg->handBackAfter == 0 ? kServeHandBackAfter : cfg->handBackAfter;
   std::vector<int> pending;  // request indices not yet admitted, in arrival order
   pending.reserve(numReqs);
   int nextArrival = 0, nSteps = 0;
@@ -1273,7 +1281,16 @@ extern "C" int dlp_multi_serve(dlp_multi *m, const dlp_serve_request *reqs, int 
         laneReq[f.graph] = f.req;
         if ((rc = lane->push(f.graph, e)) != DLP_OK) return rc;
       } else if (e <= -2 && -2 - e >= f.k && -2 - e < maxIter) {
-        carried.push_back({f.graph, f.req, -2 - e});
+        const int k = -2 - e;
+        if (heavy > 0 && longAfter >= 0 && k >= longAfter) {
+          /* a long-lived frontier: the lane's owner-mode kernels finish it
+           * (a slice stop leaves the state a hand-back stop does) */
+          flying[f.graph] = 0;
+          laneReq[f.graph] = f.req;
+          if ((rc = lane->push(f.graph, k)) != DLP_OK) return rc;
+        } else {
+          carried.push_back({f.graph, f.req, k});
+        }
       } else {
         dlp_set_errorf_("graph %d: propagation launch exit %d from iteration %d", f.graph, e, f.k);
         return DLP_ERR_INTERNAL;

@@ -592,16 +592,18 @@ class Multi:
 
     def serve(self, requests, maxSlots: int = 64, hbmBudgetBytes: int = 0,
               plain: bool = False, handBackLanes: int = 0, sliceIterations: int = 0,
-              sliceMs: float = 0.0):
+              sliceMs: float = 0.0, handBackAfter: int = 0):
         """requests: sequence of (graph, arrival seconds), sorted by arrival.
         Returns (completion seconds per request, steps).  sliceIterations
         -1 with sliceMs < 0 runs every step to completion (step-level
-        admission)."""
+        admission).  handBackAfter: with hand-back on, a graph still running
+        at this iteration of its batch goes to a hand-back lane (0 = 1024,
+        -1 = never)."""
         n = len(requests)
         reqs = (_lib.ServeRequest * max(n, 1))(*[_lib.ServeRequest(int(g), float(t))
                                                   for g, t in requests])
         cfg = _lib.ServeConfig(maxSlots, hbmBudgetBytes, 1 if plain else 0, handBackLanes,
-                               sliceIterations, sliceMs)
+                               sliceIterations, sliceMs, handBackAfter)
         done = np.zeros(max(n, 1), np.float64)
         steps = C.c_int()
         check(self._lib.dlp_multi_serve(self._h, reqs, n, C.byref(cfg),

@@ -65,6 +65,24 @@ def requests(tenants, rng, spread):
 ])
 def test_serving_same_alone_and_in_mix(gpu_available, tenants, slots, budget, plain, spread,
                                        heavy, lanes, slice):
+    check_served(tenants, slots, budget, plain, spread, heavy, lanes, slice)
+
+
+# after: iteration of a batch from which a sliced graph goes to a hand-back
+# lane; heavyFrontier 2**30 keeps the frontier-size hand-back out of it
+@pytest.mark.parametrize("after,lanes,slice", [
+    (16, 2, (8, -1.0)),    # batches past 16 iterations finish on two lanes
+    (1, 0, (1, -1.0)),     # every batch leaves after its first slice (default lanes)
+    (-1, 2, (8, -1.0)),    # never: every batch stays in the shared launches
+])
+def test_serving_long_batches_handed_back(gpu_available, tenants, after, lanes, slice):
+    """A batch still running at iteration `after` of its batch leaves the
+    shared launch for a hand-back lane at its next slice stop: the same
+    labels, iterations and updates as every tenant alone."""
+    check_served(tenants, 64, 0, False, 0.0, 1 << 30, lanes, slice, after)
+
+
+def check_served(tenants, slots, budget, plain, spread, heavy, lanes, slice, after=0):
     ref = alone(tenants, 3000)
     cfgs = [L.make_config(t, maxIterations=3000) for t in tenants]
     rng = np.random.default_rng(11)
@@ -77,9 +95,10 @@ def test_serving_same_alone_and_in_mix(gpu_available, tenants, slots, budget, pl
             budget = max(m.batch_scratch_bytes(t, b) for t, inp in enumerate(tenants)
                          for b in range(1, inp.numBatches))
         print(f"serve slots={slots} budget={budget} plain={plain} heavy={heavy} "
-              f"slice={slice}", flush=True)
+              f"slice={slice} after={after}", flush=True)
         done, steps = m.serve(reqs, maxSlots=slots, hbmBudgetBytes=budget, plain=plain,
-                              handBackLanes=lanes, sliceIterations=slice[0], sliceMs=slice[1])
+                              handBackLanes=lanes, sliceIterations=slice[0], sliceMs=slice[1],
+                              handBackAfter=after)
         m.synchronize()
         assert np.all(done >= np.array([r[1] for r in reqs]))
         if plain:
