@@ -35,14 +35,29 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+
+
+def _hw_queues(argv, default=16):
+    """--hw-queues N from the command line (read before argparse runs: the
+    HIP runtime takes GPU_MAX_HW_QUEUES when it starts)."""
+    for i, arg in enumerate(argv):
+        if arg == "--hw-queues" and i + 1 < len(argv):
+            return int(argv[i + 1])
+        if arg.startswith("--hw-queues="):
+            return int(arg.split("=", 1)[1])
+    return default
+
+
 # Serving finishes handed-back graphs on host threads with a stream each (8
 # by default) beside the serving loop's stream.  HIP maps a process's
-# streams onto GPU_MAX_HW_QUEUES hardware queues (4 unless set), and kernels
+# streams onto GPU_MAX_HW_QUEUES hardware queues (4 by default), and kernels
 # of streams that share a queue run one after another, so a lane's long
-# LDS-resident launch would hold up the shared serving launches queued
-# behind it.  16 queues give each of these streams its own; the variable is
-# read when the HIP runtime starts, so it is set before anything loads it.
-os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
+# LDS-resident launch holds up whatever is queued behind it and at most four
+# streams run at once.  16 queues give each of these streams its own.  Set
+# here, before anything starts the HIP runtime.
+HW_QUEUES = _hw_queues(sys.argv)
+if HW_QUEUES > 0:
+    os.environ["GPU_MAX_HW_QUEUES"] = str(HW_QUEUES)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 
@@ -475,6 +490,7 @@ def serving_leg(args, progress, sweep_rates) -> dict:
         "slots": args.slots, "hbm_budget_bytes": args.hbm_budget,
         "slice_iterations": args.slice_iterations, "slice_ms": args.slice_ms,
         "hand_back_after": args.hand_back_after,
+        "hw_queues": os.environ.get("GPU_MAX_HW_QUEUES"),
         "updates": sat_served["updates"],
         "saturated": {"served": sat_served, "plain": sat_plain,
                       "speedup_vs_plain": sat_served["updates_per_s"] /
@@ -511,7 +527,8 @@ def serving_main(args) -> int:
         "dtype": "f32",
         "data": "synthetic (reference graph generator, tenant t seed 1042+t, streamed batches)",
         "config": {"workload": leg["workload"], "slots": leg["slots"],
-                   "hbm_budget_bytes": leg["hbm_budget_bytes"], "updates": leg["updates"]},
+                   "hbm_budget_bytes": leg["hbm_budget_bytes"], "updates": leg["updates"],
+                   "hand_back_after": leg["hand_back_after"], "hw_queues": leg["hw_queues"]},
         "tuning": leg["tuning"],
         "saturated": sat,
         "rate_sweep": leg.get("rate_sweep", []),
@@ -756,6 +773,9 @@ def main() -> int:
     ap.add_argument("--graphs", type=int, default=256, help="graphs of the batched workload")
     ap.add_argument("--heavy-sweep", type=int, nargs="*", default=[-1, 65536, 16384, 4096],
                     help="batched: hand-back frontier sizes the autotuner tries")
+    ap.add_argument("--hw-queues", type=int, default=16,
+                    help="GPU_MAX_HW_QUEUES for this process (set before the HIP runtime "
+                         "starts; 0 = leave the environment's)")
     ap.add_argument("--hand-back-after", type=int, default=0,
                     help="serving: a batch still running at this iteration goes to a "
                          "hand-back lane (0 = default 1024, -1 = never)")
