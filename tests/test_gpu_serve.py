@@ -116,3 +116,17 @@ def check_served(tenants, slots, budget, plain, spread, heavy, lanes, slice, aft
                 s = m.stats(t, b)
                 assert (s["iterations"], s["updates"]) == \
                     (ref[t][1][b]["iterations"], ref[t][1][b]["updates"]), f"tenant {t} b {b}"
+
+
+def test_serving_rejects_bad_config(gpu_available, tenants):
+    """Negative limits other than -1 are refused before anything runs."""
+    from labelpropcuda_amd import _lib
+    with L.Multi(tenants[:2]) as m:
+        m.reset()
+        m.run_next([0, 1])
+        for kw in ({"handBackAfter": -2}, {"sliceIterations": -2}, {"maxSlots": 0}):
+            with pytest.raises(_lib.DynLPError):
+                m.serve([(0, 0.0)], **kw)
+        done, _ = m.serve([(0, 0.0)], handBackAfter=-1)  # still usable
+        assert done[0] >= 0.0
+
