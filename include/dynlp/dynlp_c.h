@@ -526,10 +526,11 @@ typedef struct dlp_serve_config {
                              no limit); the next step resumes it       */
   int handBackAfter;      /* with hand-back on, a graph stopped by its
                              slice at or past this iteration of its
-                             batch goes to a hand-back lane, whose
-                             owner-mode LDS kernels run a long-lived
-                             frontier (a limit cycle) faster than the
-                             shared launch (0 = 1024, -1 = never)      */
+                             batch goes to a hand-back lane if one is
+                             free: the lane's owner-mode LDS kernels
+                             run a long-lived frontier (a limit cycle)
+                             faster than the shared launch (0 = 1024,
+                             -1 = never)                               */
 } dlp_serve_config;
 
 /* Arrival-driven continuous batching with iteration-level admission:

@@ -274,6 +274,7 @@ class HandBackLane {
     std::lock_guard<std::mutex> lock(mu_);
     return inFlight_;
   }
+  int workers() const { return workers_; }
   int error() {
     std::lock_guard<std::mutex> lock(mu_);
     return rc_;
@@ -1282,9 +1283,11 @@ extern "C" int dlp_multi_serve(dlp_multi *m, const dlp_serve_request *reqs, int 
         if ((rc = lane->push(f.graph, e)) != DLP_OK) return rc;
       } else if (e <= -2 && -2 - e >= f.k && -2 - e < maxIter) {
         const int k = -2 - e;
-        if (heavy > 0 && longAfter >= 0 && k >= longAfter) {
-          /* a long-lived frontier: the lane's owner-mode kernels finish it
-           * (a slice stop leaves the state a hand-back stop does) */
+        if (heavy > 0 && longAfter >= 0 && k >= longAfter && lane->inFlight() < lane->workers()) {
+          /* a long-lived frontier, and a lane is free: the lane's owner-mode
+           * kernels finish it (a slice stop leaves the state a hand-back
+           * stop does); with every lane busy it goes on in the shared
+           * launches and may leave at a later slice stop */
           flying[f.graph] = 0;
           laneReq[f.graph] = f.req;
           if ((rc = lane->push(f.graph, k)) != DLP_OK) return rc;
