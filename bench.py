@@ -896,6 +896,7 @@ def main() -> int:
         "propagation_kernels": engines,
         "random_access": ra_out.get("random_access"),
         "tuning": tuning,
+        "hw_queues": os.environ.get("GPU_MAX_HW_QUEUES"),
     }
     if args.grid_sync:
         out["grid_sync"] = grid_sync_costs()
