@@ -20,6 +20,9 @@ tenants (10,000-5,000,000 vertices) and 128 update batches all arriving at
 t = 0, served with coalesced steps against one batch at a time through the
 plain path, every tenant's labels compared bit for bit.
 
+The process runs with GPU_MAX_HW_QUEUES=16 (--hw-queues) so that the
+serving legs' hand-back lanes each get a hardware queue of their own.
+
 Prints ONE JSON line.  --gpus N with N > 1 exits: this build runs on one GPU.
 """
 from __future__ import annotations
@@ -34,7 +37,6 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
-
 
 
 def _hw_queues(argv, default=16):
