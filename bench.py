@@ -20,7 +20,7 @@ tenants (10,000-5,000,000 vertices) and 128 update batches all arriving at
 t = 0, served with coalesced steps against one batch at a time through the
 plain path, every tenant's labels compared bit for bit.
 
-The process runs with GPU_MAX_HW_QUEUES=16 (--hw-queues) so that the
+The process runs with GPU_MAX_HW_QUEUES=24 (--hw-queues) so that the
 serving legs' hand-back lanes each get a hardware queue of their own.
 
 Prints ONE JSON line.  --gpus N with N > 1 exits: this build runs on one GPU.
@@ -39,7 +39,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 
-def _hw_queues(argv, default=16):
+def _hw_queues(argv, default=24):
     """--hw-queues N from the command line (read before argparse runs: the
     HIP runtime takes GPU_MAX_HW_QUEUES when it starts)."""
     for i, arg in enumerate(argv):
@@ -50,12 +50,12 @@ def _hw_queues(argv, default=16):
     return default
 
 
-# Serving finishes handed-back graphs on host threads with a stream each (8
+# Serving finishes handed-back graphs on host threads with a stream each (16
 # by default) beside the serving loop's stream.  HIP maps a process's
 # streams onto GPU_MAX_HW_QUEUES hardware queues (4 by default), and kernels
 # of streams that share a queue run one after another, so a lane's long
 # LDS-resident launch holds up whatever is queued behind it and at most four
-# streams run at once.  16 queues give each of these streams its own.  Set
+# streams run at once.  24 queues give each of these streams its own.  Set
 # here, before anything starts the HIP runtime.
 HW_QUEUES = _hw_queues(sys.argv)
 if HW_QUEUES > 0:
@@ -775,7 +775,7 @@ def main() -> int:
     ap.add_argument("--graphs", type=int, default=256, help="graphs of the batched workload")
     ap.add_argument("--heavy-sweep", type=int, nargs="*", default=[-1, 65536, 16384, 4096],
                     help="batched: hand-back frontier sizes the autotuner tries")
-    ap.add_argument("--hw-queues", type=int, default=16,
+    ap.add_argument("--hw-queues", type=int, default=24,
                     help="GPU_MAX_HW_QUEUES for this process (set before the HIP runtime "
                          "starts; 0 = leave the environment's)")
     ap.add_argument("--hand-back-after", type=int, default=0,

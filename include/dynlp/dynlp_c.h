@@ -518,7 +518,7 @@ typedef struct dlp_serve_config {
   int handBackLanes;      /* host threads (each with a stream) that
                              finish handed-back graphs (multi tuning
                              heavyFrontier > 0) beside later steps;
-                             0 = 8                                     */
+                             0 = 16                                    */
   int sliceIterations;    /* a step stops each graph after this many
                              iterations (0 = 2048, -1 = no limit) ...  */
   double sliceMs;         /* ... or after this many milliseconds of its

@@ -1104,6 +1104,10 @@ constexpr double kServeSliceMs = 2.0;
  * kernels iterate several times faster than a work-group of the shared
  * launch in queue mode */
 constexpr int kServeHandBackAfter = 1024;
+/* default dlp_serve_config::handBackLanes: long batches (limit cycles) keep
+ * a lane each for their whole run, so serving wants more lanes than the
+ * lockstep runner */
+constexpr int kServeLanes = 16;
 
 /* a batch in the shared propagation launches of dlp_multi_serve */
 struct Flight {
@@ -1161,7 +1165,7 @@ extern "C" int dlp_multi_serve(dlp_multi *m, const dlp_serve_request *reqs, int 
   std::unique_ptr<HandBackLane> lane;
   const int heavy = cfg->plain ? 0 : heavy_serving(m);
   if (heavy > 0) {
-    const int workers = cfg->handBackLanes > 0 ? cfg->handBackLanes : 8;
+    const int workers = cfg->handBackLanes > 0 ? cfg->handBackLanes : kServeLanes;
     lane.reset(new HandBackLane(m, workers, true, [&](int g) {
       completion[laneReq[g]] = now();
       return_scratch(m, g);
